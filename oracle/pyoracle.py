@@ -1,0 +1,273 @@
+"""ctypes binding of the FP64 oracle (oracle/liblt_oracle.so) and of the reference-built checker
+(oracle/_ref/libltref.so). TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and the
+CPU-baseline leg of bench.py, never by the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_PKG = os.path.join(os.path.dirname(_HERE), "lenstool-hpc_amd")
+if _PKG not in sys.path:
+    sys.path.insert(0, _PKG)
+
+import ltmodel as M  # noqa: E402
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(_HERE, "liblt_oracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: run `make -C oracle` (or __graft_entry__.build())")
+        L = C.CDLL(path)
+        P = C.POINTER
+        dp = P(C.c_double)
+        soa = P(M.LtPotentialSOA)
+        gp = P(M.LtGridParam)
+        L.orc_point_gradient.argtypes = [P(M.LtPoint), P(M.LtPoint), soa, C.c_int]
+        L.orc_point_gradient2.argtypes = [P(M.LtMatrix), P(M.LtPoint), soa, C.c_int]
+        L.orc_point_kmap.argtypes = [P(M.LtMatrix), P(M.LtPoint), soa, C.c_int]
+        L.orc_point_potential.argtypes = [dp, P(M.LtPoint), soa, C.c_int]
+        L.orc_gradient_grid.argtypes = [dp, dp, gp, soa, C.c_int, C.c_double, C.c_double,
+                                        C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_gradient_abs_planes.argtypes = [dp, dp, gp, soa, C.c_int, C.c_double, C.c_double,
+                                              C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_gradient2_grid.argtypes = [P(M.LtMatrix), gp, soa, C.c_int, C.c_double, C.c_double,
+                                         C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_gradient2_planes.argtypes = [dp] * 7 + [gp, soa, C.c_int, C.c_double, C.c_double,
+                                                      C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                                      C.c_int]
+        L.orc_gradient_grid_p.argtypes = [dp, dp, gp, soa, C.c_int, C.c_double, C.c_double,
+                                          C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_potential_grid_p.argtypes = [dp, gp, soa, C.c_int, C.c_double, C.c_double, C.c_int,
+                                           C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_point_gradient_ld.argtypes = [P(M.LtPoint), P(M.LtPoint), soa, C.c_int]
+        L.orc_point_gradient2_ld.argtypes = [P(M.LtMatrix), P(M.LtPoint), soa, C.c_int]
+        L.orc_point_potential_ld.argtypes = [dp, P(M.LtPoint), soa, C.c_int]
+        L.orc_potential_grid.argtypes = [dp, gp, soa, C.c_int, C.c_double, C.c_double, C.c_int,
+                                         C.c_int, C.c_int, C.c_int]
+        L.orc_amplif_grid.argtypes = [dp, gp, soa, C.c_int, C.c_int, C.c_int]
+        L.orc_map_epilogue.argtypes = [C.c_int, C.c_int] + [dp] * 8 + [C.c_long] + [C.c_double] * 6
+        for f in ("orc_cosmo_observer_object",):
+            getattr(L, f).argtypes = [C.c_double, P(M.LtCosmoParam)]
+            getattr(L, f).restype = C.c_double
+        for f in ("orc_cosmo_object_object", "orc_cosmo_dlsds"):
+            getattr(L, f).argtypes = [C.c_double, C.c_double, P(M.LtCosmoParam)]
+            getattr(L, f).restype = C.c_double
+        gal = P(M.LtGalaxy)
+        L.orc_delense_barycenter.argtypes = [P(M.LtPoint), P(C.c_int), P(C.c_int), C.c_int, C.c_int,
+                                             gp, gal, dp, dp]
+        L.orc_source_barycenters.argtypes = [gal, C.c_int, P(C.c_int), gal, soa, C.c_int]
+        L.orc_chi_computation.argtypes = [dp, P(C.c_int), P(C.c_int), P(M.LtPoint), P(C.c_int), gal,
+                                          C.c_int]
+        L.orc_chi_barycentersource.argtypes = [dp, P(C.c_int), C.c_int, C.c_int, C.c_int, soa, gp,
+                                               P(C.c_int), gal]
+        L.orc_cond_planes.argtypes = [dp, dp, gp, soa, C.c_int, C.c_double, C.c_double, C.c_int,
+                                      C.c_int, C.c_int, C.c_int]
+        L.orc_point_cond.argtypes = [dp, dp, P(M.LtPoint), soa, C.c_int]
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_get_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def ref_lib():
+    """The reference's own CPU functions (oracle/_ref), or None when they were not built."""
+    global _ref
+    if _ref is None:
+        path = os.path.join(_HERE, "_ref", "libltref.so")
+        if not os.path.exists(path):
+            return None
+        L = C.CDLL(path)
+        P = C.POINTER
+        dp = P(C.c_double)
+        L.ref_gradient2_grid.argtypes = [P(M.LtMatrix), P(M.LtGridParam), P(M.LtPotentialSOA), C.c_int,
+                                         C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.ref_point_gradient2.argtypes = [P(M.LtMatrix), P(M.LtPoint), P(M.LtPotentialSOA), C.c_int]
+        L.ref_delense_barycenter.argtypes = [P(M.LtPoint), P(C.c_int), P(C.c_int), C.c_int, C.c_int,
+                                             P(M.LtGridParam), P(M.LtGalaxy), dp, dp]
+        L.ref_chi_computation.argtypes = [dp, P(C.c_int), P(C.c_int), P(M.LtPoint), P(C.c_int),
+                                          P(M.LtGalaxy), C.c_int]
+        L.ref_set_threads.argtypes = [C.c_int]
+        _ref = L
+    return _ref
+
+
+def _chk(st, what):
+    if st != 0:
+        raise RuntimeError(f"oracle {what} failed with status {st}")
+
+
+def set_threads(n: int):
+    lib().orc_set_threads(int(n))
+
+
+def get_threads() -> int:
+    return int(lib().orc_get_threads())
+
+
+def point_gradient(lens: M.LensModel, x: float, y: float, extended=False):
+    p = M.LtPoint(x, y)
+    out = M.LtPoint()
+    f = lib().orc_point_gradient_ld if extended else lib().orc_point_gradient
+    _chk(f(C.byref(out), C.byref(p), C.byref(lens.soa()), lens.n), "gradient")
+    return out.x, out.y
+
+
+def point_hessian(lens: M.LensModel, x: float, y: float, kmap=False, extended=False):
+    p = M.LtPoint(x, y)
+    out = M.LtMatrix()
+    if kmap:
+        f = lib().orc_point_kmap
+    else:
+        f = lib().orc_point_gradient2_ld if extended else lib().orc_point_gradient2
+    _chk(f(C.byref(out), C.byref(p), C.byref(lens.soa()), lens.n), "hessian")
+    return out.a, out.b, out.c, out.d
+
+
+def point_potential(lens: M.LensModel, x: float, y: float, extended=False):
+    p = M.LtPoint(x, y)
+    out = C.c_double()
+    f = lib().orc_point_potential_ld if extended else lib().orc_point_potential
+    _chk(f(C.byref(out), C.byref(p), C.byref(lens.soa()), lens.n), "potential")
+    return out.value
+
+
+def gradient_grid(lens: M.LensModel, grid: M.Grid, istart=0, jstart=0, with_abs=False, extended=False):
+    gx = np.empty(grid.npix)
+    gy = np.empty(grid.npix)
+    _chk(lib().orc_gradient_grid_p(M.dptr(gx), M.dptr(gy), C.byref(grid.frame), C.byref(lens.soa()),
+                                   lens.n, grid.dx, grid.dy, grid.nx, grid.ny, istart, jstart,
+                                   int(extended)), "grid")
+    if not with_abs:
+        return gx.reshape(grid.ny, grid.nx), gy.reshape(grid.ny, grid.nx)
+    ax = np.empty(grid.npix)
+    ay = np.empty(grid.npix)
+    _chk(lib().orc_gradient_abs_planes(M.dptr(ax), M.dptr(ay), C.byref(grid.frame),
+                                       C.byref(lens.soa()), lens.n, grid.dx, grid.dy, grid.nx,
+                                       grid.ny, istart, jstart), "abs")
+    s = (grid.ny, grid.nx)
+    return gx.reshape(s), gy.reshape(s), ax.reshape(s), ay.reshape(s)
+
+
+def hessian_planes(lens: M.LensModel, grid: M.Grid, istart=0, jstart=0, kmap=False, with_abs=False,
+                   extended=False):
+    n = grid.npix
+    a, b, c, d = (np.empty(n) for _ in range(4))
+    nul = C.POINTER(C.c_double)()
+    if with_abs:
+        aa, ab, ac = (np.empty(n) for _ in range(3))
+        ptrs = [M.dptr(aa), M.dptr(ab), M.dptr(ac)]
+    else:
+        ptrs = [nul, nul, nul]
+    _chk(lib().orc_gradient2_planes(M.dptr(a), M.dptr(b), M.dptr(c), M.dptr(d), *ptrs,
+                                    C.byref(grid.frame), C.byref(lens.soa()), lens.n, grid.dx,
+                                    grid.dy, grid.nx, grid.ny, istart, jstart, int(kmap), int(extended)),
+         "hessian grid")
+    s = (grid.ny, grid.nx)
+    out = [a.reshape(s), b.reshape(s), c.reshape(s), d.reshape(s)]
+    if with_abs:
+        out += [aa.reshape(s), ab.reshape(s), ac.reshape(s)]
+    return out
+
+
+def potential_grid(lens: M.LensModel, grid: M.Grid, istart=0, jstart=0, extended=False):
+    phi = np.empty(grid.npix)
+    _chk(lib().orc_potential_grid_p(M.dptr(phi), C.byref(grid.frame), C.byref(lens.soa()), lens.n,
+                                    grid.dx, grid.dy, grid.nx, grid.ny, istart, jstart, int(extended)),
+         "potential grid")
+    return phi.reshape(grid.ny, grid.nx)
+
+
+def cond_planes(lens: M.LensModel, grid: M.Grid, istart=0, jstart=0):
+    """FP64 rounding-error scale of the closed forms: (gradient, Hessian) planes, absolute."""
+    cg = np.empty(grid.npix)
+    ch = np.empty(grid.npix)
+    _chk(lib().orc_cond_planes(M.dptr(cg), M.dptr(ch), C.byref(grid.frame), C.byref(lens.soa()), lens.n,
+                               grid.dx, grid.dy, grid.nx, grid.ny, istart, jstart), "cond")
+    s = (grid.ny, grid.nx)
+    return cg.reshape(s), ch.reshape(s)
+
+
+def point_cond(lens: M.LensModel, x, y):
+    cg = C.c_double()
+    ch = C.c_double()
+    p = M.LtPoint(x, y)
+    _chk(lib().orc_point_cond(C.byref(cg), C.byref(ch), C.byref(p), C.byref(lens.soa()), lens.n), "cond")
+    return cg.value, ch.value
+
+
+def map_epilogue(kind: int, mode: int, a, b, c, dl0s, dos, dol, h, dx, dy, nout=1):
+    a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+    b = np.ascontiguousarray(b, dtype=np.float64).ravel()
+    c = np.ascontiguousarray(c, dtype=np.float64).ravel()
+    outs = [np.empty_like(a) for _ in range(nout)]
+    nul = C.POINTER(C.c_double)()
+    ptrs = [M.dptr(o) for o in outs] + [nul] * (5 - nout)
+    _chk(lib().orc_map_epilogue(kind, mode, *ptrs, M.dptr(a), M.dptr(b), M.dptr(c), a.size,
+                                dl0s, dos, dol, h, dx, dy), "epilogue")
+    return outs
+
+
+def cosmo_dlsds(zl, zs, cosmo):
+    return lib().orc_cosmo_dlsds(zl, zs, C.byref(cosmo))
+
+
+def cosmo_observer_object(z, cosmo):
+    return lib().orc_cosmo_observer_object(z, C.byref(cosmo))
+
+
+def cosmo_object_object(z1, z2, cosmo):
+    return lib().orc_cosmo_object_object(z1, z2, C.byref(cosmo))
+
+
+def delense_barycenter(nsets, n, frame, sources: np.ndarray, gx, gy):
+    pos = np.zeros((nsets, M.MAXIMPERSOURCE, 2))
+    loc = np.zeros(nsets, dtype=np.int32)
+    numimg = C.c_int(0)
+    gx = np.ascontiguousarray(gx, dtype=np.float64)
+    gy = np.ascontiguousarray(gy, dtype=np.float64)
+    st = lib().orc_delense_barycenter(pos.ctypes.data_as(C.POINTER(M.LtPoint)),
+                                      loc.ctypes.data_as(C.POINTER(C.c_int)), C.byref(numimg), nsets,
+                                      n, C.byref(frame), M.galaxy_ptr(sources), M.dptr(gx), M.dptr(gy))
+    return st, pos, loc, numimg.value
+
+
+def source_barycenters(nimages, images: np.ndarray, lens: M.LensModel):
+    nimages = np.ascontiguousarray(nimages, dtype=np.int32)
+    src = M.galaxies_array(len(nimages))
+    _chk(lib().orc_source_barycenters(M.galaxy_ptr(src), len(nimages),
+                                      nimages.ctypes.data_as(C.POINTER(C.c_int)),
+                                      M.galaxy_ptr(images), C.byref(lens.soa()), lens.n), "barycentres")
+    return src
+
+
+def chi_computation(loc, pos, nimages, images):
+    nimages = np.ascontiguousarray(nimages, dtype=np.int32)
+    loc = np.ascontiguousarray(loc, dtype=np.int32)
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    chi = C.c_double(0.0)
+    found = C.c_int(0)
+    _chk(lib().orc_chi_computation(C.byref(chi), C.byref(found), loc.ctypes.data_as(C.POINTER(C.c_int)),
+                                   pos.ctypes.data_as(C.POINTER(M.LtPoint)),
+                                   nimages.ctypes.data_as(C.POINTER(C.c_int)), M.galaxy_ptr(images),
+                                   len(nimages)), "chi")
+    return chi.value, found.value
+
+
+def chi_barycentersource(lens: M.LensModel, n, frame, nimages, images):
+    nimages = np.ascontiguousarray(nimages, dtype=np.int32)
+    chi = C.c_double(0.0)
+    err = C.c_int(0)
+    st = lib().orc_chi_barycentersource(C.byref(chi), C.byref(err), n, len(nimages), lens.n,
+                                        C.byref(lens.soa()), C.byref(frame),
+                                        nimages.ctypes.data_as(C.POINTER(C.c_int)), M.galaxy_ptr(images))
+    return st, chi.value
