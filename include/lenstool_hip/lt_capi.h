@@ -1,0 +1,142 @@
+/*
+ * lt_capi.h -- C-ABI of the MI355X lens-map library (liblenstool_hip.so).
+ *
+ * Plain C: pointers, sizes and the structures of lt_types.h; no C++ or HIP types in any signature
+ * (streams are passed as void*, a hipStream_t or NULL for the library's own stream). Every function
+ * returns an LT_* status (lt_status.h); lt_last_error() gives the message of the calling thread's
+ * last failure.
+ *
+ * Two families of entry points:
+ *  1. Synchronous, host-array entry points with the argument meaning of the reference's GPU
+ *     wrappers (each cites the function it replaces). Inputs are host SOA arrays, outputs are
+ *     caller-allocated host arrays of nx*ny doubles, row-major (index = row*nx + col), pixel (col, row)
+ *     at (xmin + (col+istart) dx, ymin + (row+jstart) dy). They keep device buffers cached between
+ *     calls instead of allocating and freeing every time, and are serialised by an internal lock.
+ *  2. Resident models and device-pointer entry points: a lens model is packed and uploaded once
+ *     (lt_model_create) and maps are computed asynchronously on a caller stream into device memory.
+ *
+ * precision: LT_PREC_FP64 or LT_PREC_FP32C (lt_status.h).
+ */
+#ifndef LENSTOOL_HIP_LT_CAPI_H
+#define LENSTOOL_HIP_LT_CAPI_H
+
+#include "lt_status.h"
+#include "lt_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- library ----------------------------------------------------------------------------------- */
+const char* lt_last_error(void);
+int lt_version(void);                       /* 10000*major + 100*minor + patch                  */
+int lt_device_count(int* count);            /* number of visible GPUs                           */
+int lt_init(int device);                    /* select the GPU used by the calling thread        */
+int lt_sync(void);                          /* wait for the library stream                      */
+
+/* ---- device memory, streams and events (for callers without their own HIP runtime) ------------- */
+int lt_dev_alloc(void** ptr, long long bytes);
+int lt_dev_free(void* ptr);
+int lt_memcpy_h2d(void* dst, const void* src, long long bytes, void* stream);  /* waits when stream == NULL */
+int lt_memcpy_d2h(void* dst, const void* src, long long bytes, void* stream);
+int lt_memset_d(void* dst, int value, long long bytes, void* stream);
+/* device-to-device copy done by a kernel (a streaming copy of known size, also used to calibrate
+ * hardware traffic counters) */
+int lt_memcpy_d2d(void* dst, const void* src, long long bytes, void* stream);
+int lt_stream_create(void** stream);
+int lt_stream_destroy(void* stream);
+int lt_stream_sync(void* stream);
+int lt_event_create(void** event);
+int lt_event_destroy(void* event);
+int lt_event_record(void* event, void* stream);
+int lt_event_sync(void* event);
+int lt_event_elapsed(float* ms, void* start, void* stop);
+
+/* ---- reference-compatible synchronous entry points --------------------------------------------- */
+
+/* alpha = sum of halo gradients. Replaces gradient_grid_GPU(..., dx, dy, nx, ny, istart, jstart)
+ * (src/grid_gradient_GPU.cuh:36, src/grid_gradient_GPU.cu:87-179). */
+int lt_gradient_grid(double* grid_grad_x, double* grid_grad_y, const lt_grid_param* frame,
+                     const lt_potential_soa* lens, int nhalos, double dx, double dy,
+                     int nbgridcells_x, int nbgridcells_y, int istart, int jstart, int precision);
+
+/* Hessian planes a = phi_xx, b = d = phi_xy, c = phi_yy (any plane may be NULL). Replaces
+ * gradient2_grid_GPU(a, b, c, d, ..., dx, dy, nx, ny, istart, jstart)
+ * (src/grid_gradient2_GPU.cuh:34, src/grid_gradient2_GPU.cu:67-183). */
+int lt_gradient2_grid(double* a, double* b, double* c, double* d, const lt_grid_param* frame,
+                      const lt_potential_soa* lens, int nhalos, double dx, double dy,
+                      int nbgridcells_x, int nbgridcells_y, int istart, int jstart, int precision);
+
+/* Hessian with every (type, dlsds) run weighted by lens->dlsds. Replaces
+ * module_potentialDerivatives_Kmap_SOA_CPU_GPU (src/grid_gradient2_GPU.cuh:43,
+ * src/grid_gradient2_GPU.cu:212-236). */
+int lt_kmap_grid(double* a, double* b, double* c, double* d, const lt_grid_param* frame,
+                 const lt_potential_soa* lens, int nhalos, double dx, double dy,
+                 int nbgridcells_x, int nbgridcells_y, int istart, int jstart, int precision);
+
+/* Lensing potential. Replaces potential_grid_GPU(..., dx, dy, nx, ny, istart, jstart)
+ * (src/grid_potential_GPU.cuh:31); types 5, 8 and 81 (the reference has 81 only). FP64. */
+int lt_potential_grid(double* grid_potential, const lt_grid_param* frame, const lt_potential_soa* lens,
+                      int nhalos, double dx, double dy, int nbgridcells_x, int nbgridcells_y,
+                      int istart, int jstart);
+
+/* One map of the reference runmode families, Hessian (or gradient / potential) and epilogue fused
+ * in one kernel. Replaces
+ *   map_grid_ampli_GPU (src/grid_map_ampli_GPU.cuh:51)   kind LT_MAP_AMPLI, modes 1..6
+ *   map_grid_shear_GPU (src/grid_map_shear_GPU.cuh:49)   kind LT_MAP_SHEAR, modes 1..4
+ *   map_grid_mass_GPU  (src/grid_map_mass_GPU.cuh:48)    kind LT_MAP_MASS,  modes 1..4 (Kmap)
+ *   map_grid_potential_GPU (src/grid_map_pot_GPU.cuh:48) kind LT_MAP_POT,   modes 1..3
+ *   map_grid_dpl_GPU   (src/grid_map_dpl_GPU.cuh:45)     kind LT_MAP_DPL,   map0 = x, map1 = y
+ * z_lens is the lens plane (the reference passes lens->z[0]; for LT_MAP_MASS 0 means lens->z[0]),
+ * z_source the source plane. map1 is used by LT_MAP_DPL only. */
+int lt_map_grid(int kind, int mode, double* map0, double* map1, const lt_cosmo_param* cosmo,
+                const lt_grid_param* frame, const lt_potential_soa* lens, int nhalos, double z_lens,
+                double z_source, double dx, double dy, int nbgridcells_x, int nbgridcells_y, int istart,
+                int jstart, int precision);
+
+/* kappa, gamma1, gamma2, det A = (1-kappa)^2 - gamma^2 and mu = 1/det A in one pass (any output may
+ * be NULL), for the source plane z_source behind a lens at z_lens. No reference counterpart: the
+ * reference computes each of these with its own Hessian launch (src/grid_map_ampli_GPU.cu:93-192). */
+int lt_map_fields(double* kappa, double* gamma1, double* gamma2, double* det_a, double* mu,
+                  const lt_cosmo_param* cosmo, const lt_grid_param* frame, const lt_potential_soa* lens,
+                  int nhalos, double z_lens, double z_source, double dx, double dy, int nbgridcells_x,
+                  int nbgridcells_y, int istart, int jstart, int precision);
+
+/* ---- cosmology (src/module_cosmodistances.cpp:94-174), distances in units of c/H0 -------------- */
+double lt_cosmo_observer_object(double z, const lt_cosmo_param* cosmo);
+double lt_cosmo_object_object(double z1, double z2, const lt_cosmo_param* cosmo);
+double lt_cosmo_dlsds(double z_lens, double z_source, const lt_cosmo_param* cosmo);
+
+/* ---- resident models and device-pointer entry points ------------------------------------------- */
+typedef struct lt_model_s* lt_model;
+
+/* Pack and upload a lens model; use = LT_MAP_GRADIENT, LT_MAP_HESSIAN (also maps of kinds AMPLI,
+ * SHEAR, FIELDS), LT_MAP_MASS (Kmap runs) or LT_MAP_POT. */
+int lt_model_create(lt_model* out, const lt_potential_soa* lens, int nhalos, int use);
+/* Replace the parameters of a model in place (same halo count and types), e.g. per Bayes sample;
+ * ordered on `stream` after the work already queued there. */
+int lt_model_update(lt_model model, const lt_potential_soa* lens, int nhalos, void* stream);
+int lt_model_destroy(lt_model model);
+/* device bytes held by a model */
+int lt_model_bytes(lt_model model, long long* bytes);
+
+/* Grid of a device map. pitch = elements between rows (>= nx). */
+typedef struct lt_grid_desc {
+    double xmin, ymin, dx, dy;
+    int nx, ny, istart, jstart;
+    long long pitch;
+} lt_grid_desc;
+
+/* Compute one map into device arrays on `stream` (asynchronous). kind/mode as lt_map_grid plus
+ * LT_MAP_HESSIAN (out0..out3 = a, b, c, d), LT_MAP_GRADIENT (out0, out1 = x, y) and LT_MAP_FIELDS
+ * (out0..out4 = kappa, gamma1, gamma2, det A, mu). r = D_ls/D_s used by the map formulas, mult = the
+ * mass / potential multiplier of the mode. px = pixels per thread (0 = the tuned default). */
+int lt_map_device(lt_model model, int kind, int mode, double* out0, double* out1, double* out2, double* out3,
+                  double* out4, double r, double mult, const lt_grid_desc* grid, int precision, int px,
+                  void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LENSTOOL_HIP_LT_CAPI_H */

@@ -1,0 +1,641 @@
+// lt_capi.hip -- implementation of the C-ABI (include/lenstool_hip/lt_capi.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "lenstool_hip/lt_capi.h"
+#include "lt_kernels.h"
+
+using namespace lt;
+
+namespace {
+
+thread_local std::string g_err;
+thread_local int g_device = -1;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* where)
+{
+    std::string m = std::string(where) + ": " + hipGetErrorString(e);
+    return fail(LT_ERR_HIP, m);
+}
+
+#define HIP_TRY(expr)                                         \
+    do {                                                      \
+        hipError_t e_ = (expr);                               \
+        if (e_ != hipSuccess) return hip_fail(e_, #expr);     \
+    } while (0)
+
+// Per-device state of the synchronous host-array entry points: a stream and grow-only buffers.
+struct DeviceState {
+    std::mutex mu;
+    hipStream_t stream = nullptr;
+    void* halos = nullptr;
+    size_t halos_bytes = 0;
+    void* planes = nullptr;
+    size_t planes_bytes = 0;
+};
+
+constexpr int kMaxDevices = 64;
+DeviceState g_state[kMaxDevices];
+
+int current_device(int* dev)
+{
+    if (g_device < 0) {
+        int n = 0;
+        hipError_t e = hipGetDeviceCount(&n);
+        if (e != hipSuccess || n <= 0) return fail(LT_ERR_NO_DEVICE, "no GPU visible to the HIP runtime");
+        HIP_TRY(hipGetDevice(&g_device));
+    }
+    if (g_device >= kMaxDevices) return fail(LT_ERR_NO_DEVICE, "device index out of range");
+    HIP_TRY(hipSetDevice(g_device));
+    *dev = g_device;
+    return LT_OK;
+}
+
+int ensure(void** buf, size_t* have, size_t need)
+{
+    if (need <= *have) return LT_OK;
+    if (*buf) HIP_TRY(hipFree(*buf));
+    *buf = nullptr;
+    *have = 0;
+    size_t n = need + need / 4;
+    hipError_t e = hipMalloc(buf, n);
+    if (e != hipSuccess) {
+        *buf = nullptr;
+        (void)hipGetLastError();
+        return fail(LT_ERR_NOMEM, "device allocation of " + std::to_string(n) + " bytes failed");
+    }
+    *have = n;
+    return LT_OK;
+}
+
+int default_px(int kind, int precision)
+{
+    (void)kind;
+    (void)precision;
+    return 2;
+}
+
+struct DevModel {
+    HaloRec* recs = nullptr;
+    HaloRec32* recs32 = nullptr;
+    Run* runs = nullptr;
+    int nruns = 0;
+};
+
+// Launch the kernel that produces map `kind` from a device model.
+int launch_map(const DevModel& m, int kind, int mode, double* const outs[5], double r, double mult,
+               const GridDesc& g, int precision, int px, hipStream_t s)
+{
+    OutDesc o{};
+    o.p0 = outs[0];
+    o.p1 = outs[1];
+    o.p2 = outs[2];
+    o.p3 = outs[3];
+    o.p4 = outs[4];
+    o.kind = kind;
+    o.mode = mode;
+    o.r = r;
+    o.mult = mult;
+    if (px <= 0) px = default_px(kind, precision);
+    if (px != 1 && px != 2 && px != 4) return fail(LT_ERR_INVALID, "px must be 0, 1, 2 or 4");
+    if (precision != LT_PREC_FP64)
+        return fail(LT_ERR_INVALID, "precision: only LT_PREC_FP64 is available in this build");
+    hipError_t e;
+    switch (kind) {
+    case LT_MAP_GRADIENT:
+    case LT_MAP_DPL:
+        e = launch_gradient64(m.recs, m.runs, m.nruns, g, o, px, s);
+        break;
+    case LT_MAP_POT:
+        e = launch_potential64(m.recs, m.runs, m.nruns, g, o, s);
+        break;
+    default:
+        e = launch_hessian64(m.recs, m.runs, m.nruns, g, o, px, s);
+        break;
+    }
+    if (e != hipSuccess) return hip_fail(e, "kernel launch");
+    return LT_OK;
+}
+
+int planes_for_kind(int kind)
+{
+    switch (kind) {
+    case LT_MAP_HESSIAN: return 4;
+    case LT_MAP_FIELDS: return 5;
+    case LT_MAP_GRADIENT:
+    case LT_MAP_DPL: return 2;
+    default: return 1;
+    }
+}
+
+int check_kind_mode(int kind, int mode)
+{
+    switch (kind) {
+    case LT_MAP_AMPLI: if (mode >= 1 && mode <= 6) return LT_OK; break;
+    case LT_MAP_SHEAR: if (mode >= 1 && mode <= 4) return LT_OK; break;
+    case LT_MAP_MASS: if (mode >= 1 && mode <= 4) return LT_OK; break;
+    case LT_MAP_POT: if (mode >= 1 && mode <= 3) return LT_OK; break;
+    case LT_MAP_DPL:
+    case LT_MAP_FIELDS:
+    case LT_MAP_HESSIAN:
+    case LT_MAP_GRADIENT: return LT_OK;
+    default: return fail(LT_ERR_INVALID, "unknown map kind " + std::to_string(kind));
+    }
+    return fail(LT_ERR_INVALID, "mode " + std::to_string(mode) + " not valid for map kind " + std::to_string(kind));
+}
+
+int check_grid(const lt_grid_param* frame, double dx, double dy, int nx, int ny)
+{
+    if (!frame) return fail(LT_ERR_INVALID, "frame is NULL");
+    if (nx <= 0 || ny <= 0) return fail(LT_ERR_INVALID, "grid size must be positive");
+    if (!std::isfinite(frame->xmin) || !std::isfinite(frame->ymin) || !std::isfinite(dx) || !std::isfinite(dy))
+        return fail(LT_ERR_INVALID, "non-finite frame or pixel size");
+    return LT_OK;
+}
+
+// Synchronous host-array path: pack, upload, launch, download.
+int host_map(int kind, int mode, Use use, double* const host_out[5], const lt_grid_param* frame,
+             const lt_potential_soa* lens, int nhalos, double r, double mult, double dx, double dy, int nx, int ny,
+             int istart, int jstart, int precision)
+{
+    int st = check_kind_mode(kind, mode);
+    if (st) return st;
+    if ((st = check_grid(frame, dx, dy, nx, ny))) return st;
+    int np = planes_for_kind(kind);
+    bool any = false;
+    for (int k = 0; k < np; ++k) any |= host_out[k] != nullptr;
+    if (!any) return fail(LT_ERR_INVALID, "no output array given");
+    if ((kind == LT_MAP_GRADIENT || kind == LT_MAP_DPL) && (!host_out[0] || !host_out[1]))
+        return fail(LT_ERR_INVALID, "gradient output needs both x and y arrays");
+    if (kind != LT_MAP_HESSIAN && kind != LT_MAP_FIELDS && kind != LT_MAP_GRADIENT && kind != LT_MAP_DPL && !host_out[0])
+        return fail(LT_ERR_INVALID, "map output array is NULL");
+
+    std::vector<HaloRec> recs;
+    std::vector<HaloRec32> recs32;
+    std::vector<Run> runs;
+    std::string err;
+    if ((st = pack_model(lens, nhalos, use, recs, recs32, runs, err))) return fail(st, err);
+
+    int dev;
+    if ((st = current_device(&dev))) return st;
+    DeviceState& S = g_state[dev];
+    std::lock_guard<std::mutex> lock(S.mu);
+    if (!S.stream) HIP_TRY(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking));
+
+    const size_t n = (size_t)nx * (size_t)ny;
+    const size_t rec_bytes = recs.size() * sizeof(HaloRec);
+    const size_t rec32_bytes = recs32.size() * sizeof(HaloRec32);
+    const size_t run_bytes = runs.size() * sizeof(Run);
+    const size_t halo_need = rec_bytes + rec32_bytes + run_bytes + 256;
+    if ((st = ensure(&S.halos, &S.halos_bytes, halo_need))) return st;
+    if ((st = ensure(&S.planes, &S.planes_bytes, n * np * sizeof(double)))) return st;
+
+    char* base = (char*)S.halos;
+    DevModel m;
+    m.recs = (HaloRec*)base;
+    m.recs32 = (HaloRec32*)(base + rec_bytes);
+    m.runs = (Run*)(base + ((rec_bytes + rec32_bytes + 63) / 64) * 64);
+    m.nruns = (int)runs.size();
+    if (rec_bytes) HIP_TRY(hipMemcpyAsync(m.recs, recs.data(), rec_bytes, hipMemcpyHostToDevice, S.stream));
+    if (rec32_bytes) HIP_TRY(hipMemcpyAsync(m.recs32, recs32.data(), rec32_bytes, hipMemcpyHostToDevice, S.stream));
+    if (run_bytes) HIP_TRY(hipMemcpyAsync(m.runs, runs.data(), run_bytes, hipMemcpyHostToDevice, S.stream));
+
+    double* dev_out[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    for (int k = 0; k < np; ++k)
+        if (host_out[k]) dev_out[k] = (double*)S.planes + k * n;
+    GridDesc g{};
+    g.xmin = frame->xmin;
+    g.ymin = frame->ymin;
+    g.dx = dx;
+    g.dy = dy;
+    g.nx = nx;
+    g.ny = ny;
+    g.istart = istart;
+    g.jstart = jstart;
+    g.pitch = nx;
+    if ((st = launch_map(m, kind, mode, dev_out, r, mult, g, precision, 0, S.stream))) return st;
+    for (int k = 0; k < np; ++k)
+        if (host_out[k])
+            HIP_TRY(hipMemcpyAsync(host_out[k], dev_out[k], n * sizeof(double), hipMemcpyDeviceToHost, S.stream));
+    HIP_TRY(hipStreamSynchronize(S.stream));
+    return LT_OK;
+}
+
+}  // namespace
+
+// =================================================================================================
+// exported C-ABI
+// =================================================================================================
+extern "C" {
+
+const char* lt_last_error(void) { return g_err.c_str(); }
+
+int lt_version(void) { return 100; }
+
+int lt_device_count(int* count)
+{
+    if (!count) return fail(LT_ERR_INVALID, "count is NULL");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    *count = n;
+    return LT_OK;
+}
+
+int lt_init(int device)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(LT_ERR_NO_DEVICE, "no GPU visible to the HIP runtime");
+    if (device < 0 || device >= n || device >= kMaxDevices) return fail(LT_ERR_INVALID, "device index out of range");
+    HIP_TRY(hipSetDevice(device));
+    g_device = device;
+    return LT_OK;
+}
+
+int lt_sync(void)
+{
+    int dev, st;
+    if ((st = current_device(&dev))) return st;
+    if (g_state[dev].stream) HIP_TRY(hipStreamSynchronize(g_state[dev].stream));
+    return LT_OK;
+}
+
+int lt_dev_alloc(void** ptr, long long bytes)
+{
+    if (!ptr || bytes < 0) return fail(LT_ERR_INVALID, "bad allocation request");
+    *ptr = nullptr;
+    int dev, st;
+    if ((st = current_device(&dev))) return st;
+    hipError_t e = hipMalloc(ptr, (size_t)(bytes > 0 ? bytes : 1));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *ptr = nullptr;
+        return fail(LT_ERR_NOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+    }
+    return LT_OK;
+}
+
+int lt_dev_free(void* ptr)
+{
+    if (ptr) HIP_TRY(hipFree(ptr));
+    return LT_OK;
+}
+
+int lt_memcpy_h2d(void* dst, const void* src, long long bytes, void* stream)
+{
+    if (bytes < 0 || (bytes && (!dst || !src))) return fail(LT_ERR_INVALID, "bad copy");
+    if (!bytes) return LT_OK;
+    if (stream) HIP_TRY(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+    else HIP_TRY(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice));
+    return LT_OK;
+}
+
+int lt_memcpy_d2h(void* dst, const void* src, long long bytes, void* stream)
+{
+    if (bytes < 0 || (bytes && (!dst || !src))) return fail(LT_ERR_INVALID, "bad copy");
+    if (!bytes) return LT_OK;
+    if (stream) HIP_TRY(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
+    else HIP_TRY(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost));
+    return LT_OK;
+}
+
+int lt_memset_d(void* dst, int value, long long bytes, void* stream)
+{
+    if (bytes < 0 || (bytes && !dst)) return fail(LT_ERR_INVALID, "bad memset");
+    if (!bytes) return LT_OK;
+    HIP_TRY(hipMemsetAsync(dst, value, (size_t)bytes, (hipStream_t)stream));
+    if (!stream) HIP_TRY(hipStreamSynchronize(nullptr));
+    return LT_OK;
+}
+
+int lt_memcpy_d2d(void* dst, const void* src, long long bytes, void* stream)
+{
+    if (bytes < 0 || (bytes && (!dst || !src))) return fail(LT_ERR_INVALID, "bad copy");
+    if (!bytes) return LT_OK;
+    hipError_t e = launch_copy(dst, src, (size_t)bytes, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "copy kernel");
+    if (!stream) HIP_TRY(hipStreamSynchronize(nullptr));
+    return LT_OK;
+}
+
+int lt_stream_create(void** stream)
+{
+    if (!stream) return fail(LT_ERR_INVALID, "stream is NULL");
+    int dev, st;
+    if ((st = current_device(&dev))) return st;
+    hipStream_t s;
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = s;
+    return LT_OK;
+}
+
+int lt_stream_destroy(void* stream)
+{
+    if (stream) HIP_TRY(hipStreamDestroy((hipStream_t)stream));
+    return LT_OK;
+}
+
+int lt_stream_sync(void* stream)
+{
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return LT_OK;
+}
+
+int lt_event_create(void** event)
+{
+    if (!event) return fail(LT_ERR_INVALID, "event is NULL");
+    int dev, st;
+    if ((st = current_device(&dev))) return st;
+    hipEvent_t e;
+    HIP_TRY(hipEventCreate(&e));
+    *event = e;
+    return LT_OK;
+}
+
+int lt_event_destroy(void* event)
+{
+    if (event) HIP_TRY(hipEventDestroy((hipEvent_t)event));
+    return LT_OK;
+}
+
+int lt_event_record(void* event, void* stream)
+{
+    HIP_TRY(hipEventRecord((hipEvent_t)event, (hipStream_t)stream));
+    return LT_OK;
+}
+
+int lt_event_sync(void* event)
+{
+    HIP_TRY(hipEventSynchronize((hipEvent_t)event));
+    return LT_OK;
+}
+
+int lt_event_elapsed(float* ms, void* start, void* stop)
+{
+    if (!ms) return fail(LT_ERR_INVALID, "ms is NULL");
+    HIP_TRY(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+    return LT_OK;
+}
+
+double lt_cosmo_observer_object(double z, const lt_cosmo_param* c) { return c ? cosmo_observer_object(z, *c) : NAN; }
+
+double lt_cosmo_object_object(double z1, double z2, const lt_cosmo_param* c)
+{
+    return c ? cosmo_object_object(z1, z2, *c) : NAN;
+}
+
+double lt_cosmo_dlsds(double zl, double zs, const lt_cosmo_param* c)
+{
+    if (!c) return NAN;
+    if (zl >= zs) return 0.0;
+    return cosmo_object_object(zl, zs, *c) / cosmo_observer_object(zs, *c);
+}
+
+int lt_gradient_grid(double* gx, double* gy, const lt_grid_param* frame, const lt_potential_soa* lens, int nhalos,
+                     double dx, double dy, int nx, int ny, int istart, int jstart, int precision)
+{
+    double* outs[5] = {gx, gy, nullptr, nullptr, nullptr};
+    return host_map(LT_MAP_GRADIENT, 0, Use::Gradient, outs, frame, lens, nhalos, 1.0, 1.0, dx, dy, nx, ny, istart,
+                    jstart, precision);
+}
+
+int lt_gradient2_grid(double* a, double* b, double* c, double* d, const lt_grid_param* frame,
+                      const lt_potential_soa* lens, int nhalos, double dx, double dy, int nx, int ny, int istart,
+                      int jstart, int precision)
+{
+    double* outs[5] = {a, b, c, d, nullptr};
+    return host_map(LT_MAP_HESSIAN, 0, Use::Hessian, outs, frame, lens, nhalos, 1.0, 1.0, dx, dy, nx, ny, istart,
+                    jstart, precision);
+}
+
+int lt_kmap_grid(double* a, double* b, double* c, double* d, const lt_grid_param* frame, const lt_potential_soa* lens,
+                 int nhalos, double dx, double dy, int nx, int ny, int istart, int jstart, int precision)
+{
+    double* outs[5] = {a, b, c, d, nullptr};
+    return host_map(LT_MAP_HESSIAN, 0, Use::Kmap, outs, frame, lens, nhalos, 1.0, 1.0, dx, dy, nx, ny, istart,
+                    jstart, precision);
+}
+
+int lt_potential_grid(double* phi, const lt_grid_param* frame, const lt_potential_soa* lens, int nhalos, double dx,
+                      double dy, int nx, int ny, int istart, int jstart)
+{
+    double* outs[5] = {phi, nullptr, nullptr, nullptr, nullptr};
+    return host_map(LT_MAP_POT, 3, Use::Potential, outs, frame, lens, nhalos, 1.0, 1.0, dx, dy, nx, ny, istart,
+                    jstart, LT_PREC_FP64);
+}
+
+int lt_map_grid(int kind, int mode, double* map0, double* map1, const lt_cosmo_param* cosmo,
+                const lt_grid_param* frame, const lt_potential_soa* lens, int nhalos, double z_lens, double z_source,
+                double dx, double dy, int nx, int ny, int istart, int jstart, int precision)
+{
+    int st = check_kind_mode(kind, mode);
+    if (st) return st;
+    if (kind == LT_MAP_FIELDS || kind == LT_MAP_HESSIAN || kind == LT_MAP_GRADIENT)
+        return fail(LT_ERR_INVALID, "lt_map_grid takes the reference map kinds AMPLI, SHEAR, MASS, POT, DPL");
+    if (!cosmo) return fail(LT_ERR_INVALID, "cosmo is NULL");
+    if (kind == LT_MAP_MASS && z_lens == 0.0) {
+        if (!lens || !lens->z || nhalos <= 0) return fail(LT_ERR_INVALID, "mass map: z_lens = 0 needs lens->z[0]");
+        z_lens = lens->z[0];
+    }
+    // distances as the reference map wrappers form them (src/grid_map_ampli_GPU.cu:110-112,
+    // src/grid_map_mass_GPU.cu:101-108)
+    const double dl0s = cosmo_object_object(z_lens, z_source, *cosmo);
+    const double dos = cosmo_observer_object(z_source, *cosmo);
+    const double dol = cosmo_observer_object(z_lens, *cosmo);
+    const double r = dl0s / dos;
+    double mult = 1.0;
+    const double h = cosmo->h;
+    if (kind == LT_MAP_MASS) {
+        switch (mode) {
+        case 2: mult = LT_CH4PIG * h / dol / dl0s * dos; break;           // g/cm^2
+        case 3: mult = LT_MCRIT12 / h * dx * dy * dol / dl0s * dos; break;  // 1e12 Msun/pixel
+        case 4: mult = LT_CH0_4PIG * h / dol / dl0s * dos; break;           // 1e12 Msun/kpc^2
+        default: mult = 1.0; break;
+        }
+    } else if (kind == LT_MAP_POT) {
+        mult = mode == 1 ? r : mode == 2 ? LT_CH2PIG * h / dol : 1.0;
+    }
+    double* outs[5] = {map0, kind == LT_MAP_DPL ? map1 : nullptr, nullptr, nullptr, nullptr};
+    Use use = kind == LT_MAP_DPL ? Use::Gradient : kind == LT_MAP_POT ? Use::Potential
+              : kind == LT_MAP_MASS ? Use::Kmap : Use::Hessian;
+    if (kind == LT_MAP_POT && precision != LT_PREC_FP64)
+        return fail(LT_ERR_INVALID, "the potential map is computed in FP64 only");
+    return host_map(kind, mode, use, outs, frame, lens, nhalos, r, mult, dx, dy, nx, ny, istart, jstart, precision);
+}
+
+int lt_map_fields(double* kappa, double* gamma1, double* gamma2, double* det_a, double* mu, const lt_cosmo_param* cosmo,
+                  const lt_grid_param* frame, const lt_potential_soa* lens, int nhalos, double z_lens, double z_source,
+                  double dx, double dy, int nx, int ny, int istart, int jstart, int precision)
+{
+    if (!cosmo) return fail(LT_ERR_INVALID, "cosmo is NULL");
+    const double r = z_lens >= z_source ? 0.0
+                                         : cosmo_object_object(z_lens, z_source, *cosmo) /
+                                               cosmo_observer_object(z_source, *cosmo);
+    double* outs[5] = {kappa, gamma1, gamma2, det_a, mu};
+    return host_map(LT_MAP_FIELDS, 0, Use::Hessian, outs, frame, lens, nhalos, r, 1.0, dx, dy, nx, ny, istart, jstart,
+                    precision);
+}
+
+// ---- resident models ------------------------------------------------------------------------------
+}  // extern "C"
+
+struct lt_model_s {
+    int device = 0;
+    Use use = Use::Hessian;
+    int nhalos = 0;
+    std::vector<Run> runs_host;
+    std::vector<int> types;
+    DevModel dm;
+    size_t bytes = 0;
+    void* base = nullptr;
+};
+
+static Use use_from_kind(int use)
+{
+    switch (use) {
+    case LT_MAP_GRADIENT:
+    case LT_MAP_DPL: return Use::Gradient;
+    case LT_MAP_MASS: return Use::Kmap;
+    case LT_MAP_POT: return Use::Potential;
+    default: return Use::Hessian;
+    }
+}
+
+extern "C" {
+
+int lt_model_create(lt_model* out, const lt_potential_soa* lens, int nhalos, int use)
+{
+    if (!out) return fail(LT_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    int st = check_kind_mode(use, use == LT_MAP_AMPLI || use == LT_MAP_SHEAR || use == LT_MAP_MASS ||
+                                      use == LT_MAP_POT ? 1 : 0);
+    if (st) return st;
+    std::vector<HaloRec> recs;
+    std::vector<HaloRec32> recs32;
+    std::vector<Run> runs;
+    std::string err;
+    Use u = use_from_kind(use);
+    if ((st = pack_model(lens, nhalos, u, recs, recs32, runs, err))) return fail(st, err);
+    int dev;
+    if ((st = current_device(&dev))) return st;
+    lt_model_s* m = new lt_model_s();
+    m->device = dev;
+    m->use = u;
+    m->nhalos = nhalos;
+    m->runs_host = runs;
+    for (int i = 0; i < nhalos; ++i) m->types.push_back(lens->type[i]);
+    const size_t rb = recs.size() * sizeof(HaloRec), r32 = recs32.size() * sizeof(HaloRec32);
+    const size_t off_runs = ((rb + r32 + 63) / 64) * 64;
+    m->bytes = off_runs + runs.size() * sizeof(Run) + 64;
+    hipError_t e = hipMalloc(&m->base, m->bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        delete m;
+        return fail(LT_ERR_NOMEM, "model allocation failed");
+    }
+    char* b = (char*)m->base;
+    m->dm.recs = (HaloRec*)b;
+    m->dm.recs32 = (HaloRec32*)(b + rb);
+    m->dm.runs = (Run*)(b + off_runs);
+    m->dm.nruns = (int)runs.size();
+    e = hipSuccess;
+    if (rb && e == hipSuccess) e = hipMemcpy(m->dm.recs, recs.data(), rb, hipMemcpyHostToDevice);
+    if (r32 && e == hipSuccess) e = hipMemcpy(m->dm.recs32, recs32.data(), r32, hipMemcpyHostToDevice);
+    if (!runs.empty() && e == hipSuccess)
+        e = hipMemcpy(m->dm.runs, runs.data(), runs.size() * sizeof(Run), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(m->base);
+        delete m;
+        return hip_fail(e, "model upload");
+    }
+    *out = m;
+    return LT_OK;
+}
+
+int lt_model_update(lt_model m, const lt_potential_soa* lens, int nhalos, void* stream)
+{
+    if (!m) return fail(LT_ERR_NOT_FOUND, "model is NULL");
+    if (nhalos != m->nhalos) return fail(LT_ERR_INVALID, "halo count differs from the model's");
+    for (int i = 0; i < nhalos; ++i)
+        if (!lens || !lens->type || lens->type[i] != m->types[i])
+            return fail(LT_ERR_INVALID, "halo types differ from the model's");
+    std::vector<HaloRec> recs;
+    std::vector<HaloRec32> recs32;
+    std::vector<Run> runs;
+    std::string err;
+    int st = pack_model(lens, nhalos, m->use, recs, recs32, runs, err);
+    if (st) return fail(st, err);
+    if (runs.size() != m->runs_host.size()) return fail(LT_ERR_INVALID, "run structure differs from the model's");
+    HIP_TRY(hipSetDevice(m->device));
+    hipStream_t s = (hipStream_t)stream;
+    const size_t rb = recs.size() * sizeof(HaloRec), r32 = recs32.size() * sizeof(HaloRec32);
+    if (rb) HIP_TRY(hipMemcpyAsync(m->dm.recs, recs.data(), rb, hipMemcpyHostToDevice, s));
+    if (r32) HIP_TRY(hipMemcpyAsync(m->dm.recs32, recs32.data(), r32, hipMemcpyHostToDevice, s));
+    if (!runs.empty()) HIP_TRY(hipMemcpyAsync(m->dm.runs, runs.data(), runs.size() * sizeof(Run), hipMemcpyHostToDevice, s));
+    // the host vectors die here: make the copies complete before returning
+    HIP_TRY(hipStreamSynchronize(s));
+    m->runs_host = runs;
+    return LT_OK;
+}
+
+int lt_model_destroy(lt_model m)
+{
+    if (!m) return LT_OK;
+    (void)hipSetDevice(m->device);
+    hipError_t e = hipFree(m->base);
+    delete m;
+    if (e != hipSuccess) return hip_fail(e, "model free");
+    return LT_OK;
+}
+
+int lt_model_bytes(lt_model m, long long* bytes)
+{
+    if (!m || !bytes) return fail(LT_ERR_INVALID, "NULL argument");
+    *bytes = (long long)m->bytes;
+    return LT_OK;
+}
+
+int lt_map_device(lt_model m, int kind, int mode, double* out0, double* out1, double* out2, double* out3, double* out4,
+                  double r, double mult, const lt_grid_desc* grid, int precision, int px, void* stream)
+{
+    if (!m) return fail(LT_ERR_NOT_FOUND, "model is NULL");
+    if (!grid) return fail(LT_ERR_INVALID, "grid is NULL");
+    int st = check_kind_mode(kind, mode);
+    if (st) return st;
+    Use need = kind == LT_MAP_DPL || kind == LT_MAP_GRADIENT ? Use::Gradient
+               : kind == LT_MAP_POT ? Use::Potential
+               : kind == LT_MAP_MASS ? Use::Kmap : Use::Hessian;
+    if (need != m->use && !(need == Use::Hessian && m->use == Use::Kmap))
+        return fail(LT_ERR_INVALID, "model was created for another kind of map");
+    if (grid->nx <= 0 || grid->ny <= 0 || grid->pitch < grid->nx) return fail(LT_ERR_INVALID, "bad grid");
+    double* outs[5] = {out0, out1, out2, out3, out4};
+    if ((kind == LT_MAP_GRADIENT || kind == LT_MAP_DPL) && (!out0 || !out1))
+        return fail(LT_ERR_INVALID, "gradient output needs two planes");
+    if (kind != LT_MAP_HESSIAN && kind != LT_MAP_FIELDS && !out0) return fail(LT_ERR_INVALID, "out0 is NULL");
+    GridDesc g{};
+    g.xmin = grid->xmin;
+    g.ymin = grid->ymin;
+    g.dx = grid->dx;
+    g.dy = grid->dy;
+    g.nx = grid->nx;
+    g.ny = grid->ny;
+    g.istart = grid->istart;
+    g.jstart = grid->jstart;
+    g.pitch = grid->pitch;
+    HIP_TRY(hipSetDevice(m->device));
+    return launch_map(m->dm, kind, mode, outs, r, mult, g, precision, px, (hipStream_t)stream);
+}
+
+}  // extern "C"

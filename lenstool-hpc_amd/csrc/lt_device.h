@@ -1,0 +1,88 @@
+// lt_device.h -- device-side data layout shared by the kernels and the host packer.
+//
+// A lens model lives in HBM as an array of 256-byte halo records (one per halo, in the SOA order of
+// the caller, i.e. grouped by type) plus a short array of runs (type, first halo, count, weight).
+// Every per-halo constant of the closed forms is precomputed on the host once per model, so the
+// per-(halo, pixel) work is only what depends on the pixel. All lanes of a wavefront walk the same
+// halo at the same time: the record address is wavefront-uniform and is read with scalar loads.
+#pragma once
+
+#include <stdint.h>
+
+namespace lt {
+
+// One halo, 40 doubles. Field meaning by type:
+//   all     : xh, yh (centre), cs, sn (cos/sin of the angle), hc2, hs2 (Hessian amplitude times
+//             cos/sin of twice the angle), b0, eps, rc, rcut (raw parameters, for the potential)
+//   8, 81   : PIEMD constants (see below); ampg / amph gradient / Hessian amplitudes
+//   5       : sa = 1 - eps, sb = 1 + eps, ampg = b0, amph = b0 (1 - eps)(1 + eps)
+//   14      : h14a, h14b, h14c = constant Hessian (lab frame)
+struct alignas(64) HaloRec {
+    double xh, yh, cs, sn;          // 0..3
+    double hc2, hs2;                // 4, 5
+    double kx, ky;                  // 6, 7   1/(1+e)^2, 1/(1-e)^2
+    double e2;                      // 8      2 sqrt(e)
+    double cx1, icx1, cx1sq;        // 9..11  (1-e)/(1+e), its inverse, its square
+    double ra2, rb2;                // 12, 13 rcore^2, rcut^2
+    double e2ra, e2rb;              // 14, 15 2 rcore sqrt(e), 2 rcut sqrt(e)
+    double ampg;                    // 16     gradient amplitude
+    double amph;                    // 17     Hessian amplitude
+    double hk1, hk2, hk3, hk4, hk5, hk6;  // 18..23 Hessian bracket coefficients
+    double sa, sb;                  // 24, 25 SIS (1 - e), (1 + e)
+    double h14a, h14b, h14c;        // 26..28 type-14 constant Hessian
+    double b0, eps, rc, rcut;       // 29..32 raw parameters (potential)
+    double pad[7];                  // 33..39
+};
+static_assert(sizeof(HaloRec) == 320, "halo record is 320 bytes");
+
+// A run of halos of one type (and, for the Kmap, one dlsds weight).
+struct alignas(32) Run {
+    int32_t type;
+    int32_t start;
+    int32_t count;
+    int32_t pad;
+    double weight;      // 1 except for the Kmap (dlsds of the run)
+    double pad2;
+};
+static_assert(sizeof(Run) == 32, "run is 32 bytes");
+
+// Single-precision image of a halo record for the FP32-compensated kernels: the same constants
+// rounded to float, plus the centre split into hi + lo floats so that pixel - centre is formed
+// with FP64 accuracy in FP32 arithmetic.
+struct alignas(64) HaloRec32 {
+    float xh_hi, xh_lo, yh_hi, yh_lo;   // 0..3
+    float cs, sn, c2, s2;               // 4..7
+    float kx, ky, e2, cx1;              // 8..11
+    float icx1, cx1sq, ra2, rb2;        // 12..15
+    float e2ra, e2rb, ampg, amph;       // 16..19
+    float hk1, hk2, hk3, hk4;           // 20..23
+    float hk5, hk6, sa, sb;             // 24..27
+    float h14a, h14b, h14c;             // 28..30
+    float near2;                        // 31: squared radius inside which this halo is taken in FP64
+};
+static_assert(sizeof(HaloRec32) == 128, "fp32 halo record is 128 bytes");
+
+// Grid description (pixel (col, row) of the output sits at (xmin + (col+istart) dx,
+// ymin + (row+jstart) dy); output index row*pitch + col).
+struct GridDesc {
+    double xmin, ymin, dx, dy;
+    int32_t nx, ny, istart, jstart;
+    int64_t pitch;
+};
+
+// What the epilogue writes. kind / mode follow lt_status.h (LT_MAP_*).
+struct OutDesc {
+    double* p0;
+    double* p1;
+    double* p2;
+    double* p3;
+    double* p4;
+    int32_t kind;
+    int32_t mode;
+    double r;        // dlsds = D_ls / D_s applied by the map formulas
+    double mult;     // mass-map multiplier (mode 2..4), potential multiplier
+};
+
+enum : int32_t { TYPE_SIS = 5, TYPE_PIEMD = 8, TYPE_DPIE = 81, TYPE_SHEAR = 14 };
+
+}  // namespace lt

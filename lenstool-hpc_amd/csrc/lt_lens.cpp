@@ -1,0 +1,172 @@
+// lt_lens.cpp -- packs a caller's SOA lens model (the reference's Potential_SOA layout) into the
+// device halo records and runs. All per-halo constants of the closed forms are formed here, once
+// per model, in double precision.
+#include <cmath>
+#include <cstdio>
+
+#include "lenstool_hip/lt_status.h"
+#include "lt_kernels.h"
+
+namespace lt {
+
+static bool finite(double v) { return std::isfinite(v); }
+
+static int bad(std::string& err, int i, const char* what)
+{
+    char buf[160];
+    std::snprintf(buf, sizeof buf, "halo %d: %s", i, what);
+    err = buf;
+    return LT_ERR_INVALID;
+}
+
+int pack_model(const lt_potential_soa* L, int n, Use use, std::vector<HaloRec>& recs,
+               std::vector<HaloRec32>& recs32, std::vector<Run>& runs, std::string& err)
+{
+    recs.clear();
+    recs32.clear();
+    runs.clear();
+    if (n < 0) {
+        err = "negative halo count";
+        return LT_ERR_INVALID;
+    }
+    if (n == 0) return LT_OK;
+    if (!L || !L->type || !L->position_x || !L->position_y || !L->b0 || !L->ellipticity_potential ||
+        !L->anglecos || !L->anglesin) {
+        err = "lens model: a required array (type, position_x/y, b0, ellipticity_potential, anglecos/sin) is NULL";
+        return LT_ERR_INVALID;
+    }
+    if (use == Use::Kmap && !L->dlsds) {
+        err = "lens model: dlsds is NULL (needed by the Kmap / mass map)";
+        return LT_ERR_INVALID;
+    }
+    recs.resize(n);
+    recs32.resize(n);
+    for (int i = 0; i < n; ++i) {
+        const int t = L->type[i];
+        const bool piemd = (t == TYPE_PIEMD || t == TYPE_DPIE);
+        bool ok = (t == TYPE_SIS || piemd) || (t == TYPE_SHEAR && (use == Use::Hessian || use == Use::Kmap));
+        if (!ok) {
+            char buf[160];
+            std::snprintf(buf, sizeof buf, "halo %d: type %d has no %s implementation", i, t,
+                          use == Use::Gradient ? "gradient" : use == Use::Potential ? "potential" : "Hessian");
+            err = buf;
+            return LT_ERR_TYPE;
+        }
+        HaloRec h{};
+        const double x = L->position_x[i], y = L->position_y[i], b0 = L->b0[i];
+        const double eps = L->ellipticity_potential[i];
+        const double cs = L->anglecos[i], sn = L->anglesin[i];
+        if (!finite(x) || !finite(y) || !finite(b0) || !finite(eps) || !finite(cs) || !finite(sn))
+            return bad(err, i, "non-finite parameter");
+        h.xh = x;
+        h.yh = y;
+        h.cs = cs;
+        h.sn = sn;
+        h.b0 = b0;
+        h.eps = eps;
+        const double cos2 = cs * cs - sn * sn, sin2 = 2.0 * cs * sn;
+        if (t == TYPE_SIS) {
+            if (!(std::fabs(eps) < 1.0)) return bad(err, i, "SIS ellipticity_potential must be in (-1, 1)");
+            h.sa = 1.0 - eps;
+            h.sb = 1.0 + eps;
+            h.ampg = b0;
+            h.amph = b0 * h.sa * h.sb;
+        } else if (piemd) {
+            if (!L->rcore || (t == TYPE_DPIE && !L->rcut)) {
+                err = "lens model: rcore / rcut is NULL";
+                return LT_ERR_INVALID;
+            }
+            const double rc = L->rcore[i];
+            const double rcut = (t == TYPE_DPIE) ? L->rcut[i] : 0.0;
+            if (!(eps > 0.0 && eps < 1.0)) return bad(err, i, "PIEMD ellipticity_potential must be in (0, 1)");
+            if (!(rc > 0.0) || !finite(rc)) return bad(err, i, "PIEMD rcore must be > 0");
+            if (t == TYPE_DPIE && (!finite(rcut) || !(rcut > 0.0) || rcut == rc))
+                return bad(err, i, "dPIE rcut must be > 0 and differ from rcore");
+            const double sqe = std::sqrt(eps);
+            const double cx1 = (1.0 - eps) / (1.0 + eps);
+            const double ci = 0.5 * (1.0 - eps * eps) / sqe;
+            h.rc = rc;
+            h.rcut = rcut;
+            h.kx = 1.0 / ((1.0 + eps) * (1.0 + eps));
+            h.ky = 1.0 / ((1.0 - eps) * (1.0 - eps));
+            h.e2 = 2.0 * sqe;
+            h.cx1 = cx1;
+            h.icx1 = 1.0 / cx1;
+            h.cx1sq = cx1 * cx1;
+            h.ra2 = rc * rc;
+            h.rb2 = rcut * rcut;
+            h.e2ra = 2.0 * rc * sqe;
+            h.e2rb = 2.0 * rcut * sqe;
+            const double t05 = (t == TYPE_DPIE) ? rcut / (rcut - rc) : 1.0;
+            h.ampg = b0 * t05 * ci;
+            h.amph = b0 * t05 * ci;
+            h.hk1 = cx1 * h.e2 * h.kx;
+            h.hk2 = cx1 * h.e2;
+            h.hk3 = h.e2 * cx1 * h.ky;
+            h.hk4 = h.e2 * h.icx1;
+            h.hk5 = h.icx1 * h.icx1 + h.e2 * h.e2 * h.ky;
+            h.hk6 = h.e2 * h.ky * h.icx1;
+        } else {  // external shear: 3 eps (cos 2t, sin 2t; sin 2t, -cos 2t)
+            h.h14a = 3.0 * eps * cos2;
+            h.h14b = 3.0 * eps * sin2;
+            h.h14c = -3.0 * eps * cos2;
+        }
+        h.hc2 = h.amph * cos2;
+        h.hs2 = h.amph * sin2;
+        recs[i] = h;
+
+        HaloRec32& f = recs32[i];
+        f = HaloRec32{};
+        f.xh_hi = (float)x;
+        f.xh_lo = (float)(x - (double)f.xh_hi);
+        f.yh_hi = (float)y;
+        f.yh_lo = (float)(y - (double)f.yh_hi);
+        f.cs = (float)cs;
+        f.sn = (float)sn;
+        f.c2 = (float)h.hc2;
+        f.s2 = (float)h.hs2;
+        f.kx = (float)h.kx;
+        f.ky = (float)h.ky;
+        f.e2 = (float)h.e2;
+        f.cx1 = (float)h.cx1;
+        f.icx1 = (float)h.icx1;
+        f.cx1sq = (float)h.cx1sq;
+        f.ra2 = (float)h.ra2;
+        f.rb2 = (float)h.rb2;
+        f.e2ra = (float)h.e2ra;
+        f.e2rb = (float)h.e2rb;
+        f.ampg = (float)h.ampg;
+        f.amph = (float)h.amph;
+        f.hk1 = (float)h.hk1;
+        f.hk2 = (float)h.hk2;
+        f.hk3 = (float)h.hk3;
+        f.hk4 = (float)h.hk4;
+        f.hk5 = (float)h.hk5;
+        f.hk6 = (float)h.hk6;
+        f.sa = (float)h.sa;
+        f.sb = (float)h.sb;
+        f.h14a = (float)h.h14a;
+        f.h14b = (float)h.h14b;
+        f.h14c = (float)h.h14c;
+        f.near2 = 0.f;
+    }
+    // runs: equal type (and, for the Kmap, equal dlsds); the scan never reads past n - 1
+    int s = 0;
+    while (s < n) {
+        const int t = L->type[s];
+        const double w = (use == Use::Kmap) ? L->dlsds[s] : 1.0;
+        if (!finite(w)) return bad(err, s, "non-finite dlsds");
+        int c = 1;
+        while (s + c < n && L->type[s + c] == t && (use != Use::Kmap || L->dlsds[s + c] == w)) ++c;
+        Run r{};
+        r.type = t;
+        r.start = s;
+        r.count = c;
+        r.weight = w;
+        runs.push_back(r);
+        s += c;
+    }
+    return LT_OK;
+}
+
+}  // namespace lt

@@ -1,0 +1,65 @@
+// lt_math.h -- small device math helpers for the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace lt {
+
+// 1/x in FP64: hardware reciprocal estimate refined by two Newton steps (each squares the
+// relative error), for the strictly positive, normal-range denominators of the closed forms.
+__device__ __forceinline__ double rcp64(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    return r;
+}
+
+// 1/sqrt(q) in FP64 for q > 0: hardware estimate plus two Newton steps
+// y <- y + y (1 - q y^2)/2.
+__device__ __forceinline__ double rsq64(double q)
+{
+    double y = __builtin_amdgcn_rsq(q);
+    double hq = 0.5 * q;
+    double e = __builtin_fma(-hq * y, y, 0.5);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-hq * y, y, 0.5);
+    y = __builtin_fma(y, e, y);
+    return y;
+}
+
+// Pixel coordinate exactly as the reference CPU forms it: xmin + (i * d), two roundings, no fused
+// multiply-add (src/grid_gradient_CPU.cpp:89-90).
+__device__ __forceinline__ double pixel_coord(double xmin, int i, double d)
+{
+#pragma clang fp contract(off)
+    double t = (double)i * d;
+    return xmin + t;
+}
+
+// Error-free transformations for compensated FP32 sums (Knuth two-sum, Dekker fast-two-sum).
+__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e)
+{
+#pragma clang fp contract(off)
+    s = a + b;
+    float bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+// Compensated accumulator: value = hi + lo, with the rounding error of every addition kept.
+struct CompF32 {
+    float hi, lo;
+    __device__ __forceinline__ void init() { hi = 0.f; lo = 0.f; }
+    __device__ __forceinline__ void add(float v)
+    {
+        float s, e;
+        two_sum(hi, v, s, e);
+        hi = s;
+        lo += e;
+    }
+    __device__ __forceinline__ double value() const { return (double)hi + (double)lo; }
+};
+
+}  // namespace lt

@@ -1,0 +1,43 @@
+// lt_util.hip -- small utility kernels.
+#include <hip/hip_runtime.h>
+
+#include "lt_kernels.h"
+
+namespace lt {
+
+// Grid-stride streaming copy with 16-byte vectors.
+__global__ __launch_bounds__(256) void k_copy16(uint4* __restrict__ dst, const uint4* __restrict__ src, size_t nvec)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nvec; i += stride) dst[i] = src[i];
+}
+
+// Grid-stride byte copy (unaligned buffers and the tail of aligned ones).
+__global__ __launch_bounds__(256) void k_copy1(unsigned char* __restrict__ dst, const unsigned char* __restrict__ src,
+                                               size_t n)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+static unsigned blocks_for(size_t n)
+{
+    size_t b = (n + 255) / 256;
+    if (b > 8192) b = 8192;
+    return (unsigned)(b ? b : 1);
+}
+
+hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s)
+{
+    const bool aligned = ((uintptr_t)dst % 16 == 0) && ((uintptr_t)src % 16 == 0);
+    const size_t nvec = aligned ? bytes / 16 : 0;
+    const size_t body = nvec * 16;
+    if (nvec)
+        hipLaunchKernelGGL(k_copy16, dim3(blocks_for(nvec)), dim3(256), 0, s, (uint4*)dst, (const uint4*)src, nvec);
+    if (bytes > body)
+        hipLaunchKernelGGL(k_copy1, dim3(blocks_for(bytes - body)), dim3(256), 0, s, (unsigned char*)dst + body,
+                           (const unsigned char*)src + body, bytes - body);
+    return hipGetLastError();
+}
+
+}  // namespace lt

@@ -1,0 +1,287 @@
+"""ctypes binding of the lens-map C-ABI (include/lenstool_hip/lt_capi.h, liblenstool_hip.so).
+
+This is the Python side of the drop-in boundary: every call goes to the HIP library; there is no
+CPU fallback. If the shared library is missing the first call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+import ltmodel as M
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblenstool_hip.so")
+
+LT_OK = 0
+PREC_FP64 = 0
+PREC_FP32C = 1
+MAP_AMPLI, MAP_SHEAR, MAP_MASS, MAP_POT, MAP_DPL, MAP_FIELDS, MAP_HESSIAN, MAP_GRADIENT = range(1, 9)
+
+_lib = None
+
+_dp = C.POINTER(C.c_double)
+_soa = C.POINTER(M.LtPotentialSOA)
+_gp = C.POINTER(M.LtGridParam)
+_cp = C.POINTER(M.LtCosmoParam)
+
+
+class LtGridDesc(C.Structure):
+    _fields_ = [("xmin", C.c_double), ("ymin", C.c_double), ("dx", C.c_double), ("dy", C.c_double),
+                ("nx", C.c_int), ("ny", C.c_int), ("istart", C.c_int), ("jstart", C.c_int),
+                ("pitch", C.c_longlong)]
+
+
+class LtError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"lenstool_hip status {status}: {msg}")
+        self.status = status
+
+
+# every symbol include/lenstool_hip/*.h declares, with its ctypes signature
+SIGNATURES = {
+    "lt_last_error": ([], C.c_char_p),
+    "lt_version": ([], C.c_int),
+    "lt_device_count": ([C.POINTER(C.c_int)], C.c_int),
+    "lt_init": ([C.c_int], C.c_int),
+    "lt_sync": ([], C.c_int),
+    "lt_dev_alloc": ([C.POINTER(C.c_void_p), C.c_longlong], C.c_int),
+    "lt_dev_free": ([C.c_void_p], C.c_int),
+    "lt_memcpy_h2d": ([C.c_void_p, C.c_void_p, C.c_longlong, C.c_void_p], C.c_int),
+    "lt_memcpy_d2h": ([C.c_void_p, C.c_void_p, C.c_longlong, C.c_void_p], C.c_int),
+    "lt_memset_d": ([C.c_void_p, C.c_int, C.c_longlong, C.c_void_p], C.c_int),
+    "lt_memcpy_d2d": ([C.c_void_p, C.c_void_p, C.c_longlong, C.c_void_p], C.c_int),
+    "lt_stream_create": ([C.POINTER(C.c_void_p)], C.c_int),
+    "lt_stream_destroy": ([C.c_void_p], C.c_int),
+    "lt_stream_sync": ([C.c_void_p], C.c_int),
+    "lt_event_create": ([C.POINTER(C.c_void_p)], C.c_int),
+    "lt_event_destroy": ([C.c_void_p], C.c_int),
+    "lt_event_record": ([C.c_void_p, C.c_void_p], C.c_int),
+    "lt_event_sync": ([C.c_void_p], C.c_int),
+    "lt_event_elapsed": ([C.POINTER(C.c_float), C.c_void_p, C.c_void_p], C.c_int),
+    "lt_gradient_grid": ([_dp, _dp, _gp, _soa, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                          C.c_int, C.c_int], C.c_int),
+    "lt_gradient2_grid": ([_dp, _dp, _dp, _dp, _gp, _soa, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
+                           C.c_int, C.c_int, C.c_int], C.c_int),
+    "lt_kmap_grid": ([_dp, _dp, _dp, _dp, _gp, _soa, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
+                      C.c_int, C.c_int, C.c_int], C.c_int),
+    "lt_potential_grid": ([_dp, _gp, _soa, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                           C.c_int], C.c_int),
+    "lt_map_grid": ([C.c_int, C.c_int, _dp, _dp, _cp, _gp, _soa, C.c_int, C.c_double, C.c_double, C.c_double,
+                     C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
+    "lt_map_fields": ([_dp, _dp, _dp, _dp, _dp, _cp, _gp, _soa, C.c_int, C.c_double, C.c_double, C.c_double,
+                       C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
+    "lt_cosmo_observer_object": ([C.c_double, _cp], C.c_double),
+    "lt_cosmo_object_object": ([C.c_double, C.c_double, _cp], C.c_double),
+    "lt_cosmo_dlsds": ([C.c_double, C.c_double, _cp], C.c_double),
+    "lt_model_create": ([C.POINTER(C.c_void_p), _soa, C.c_int, C.c_int], C.c_int),
+    "lt_model_update": ([C.c_void_p, _soa, C.c_int, C.c_void_p], C.c_int),
+    "lt_model_destroy": ([C.c_void_p], C.c_int),
+    "lt_model_bytes": ([C.c_void_p, C.POINTER(C.c_longlong)], C.c_int),
+    "lt_map_device": ([C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                       C.c_void_p, C.c_double, C.c_double, C.POINTER(LtGridDesc), C.c_int, C.c_int,
+                       C.c_void_p], C.c_int),
+}
+
+
+def lib():
+    """Load liblenstool_hip.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                               "(make -C lenstool-hpc_amd/csrc)")
+        L = C.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def check(status):
+    if status != LT_OK:
+        raise LtError(status, lib().lt_last_error().decode(errors="replace"))
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    check(lib().lt_device_count(C.byref(n)))
+    return n.value
+
+
+# ------------------------------------------------------------------------------------------------
+# synchronous host-array entry points
+# ------------------------------------------------------------------------------------------------
+
+def gradient_grid(lens: M.LensModel, grid: M.Grid, istart=0, jstart=0, precision=PREC_FP64):
+    gx = np.empty(grid.npix)
+    gy = np.empty(grid.npix)
+    check(lib().lt_gradient_grid(M.dptr(gx), M.dptr(gy), C.byref(grid.frame), C.byref(lens.soa()), lens.n,
+                                 grid.dx, grid.dy, grid.nx, grid.ny, istart, jstart, precision))
+    s = (grid.ny, grid.nx)
+    return gx.reshape(s), gy.reshape(s)
+
+
+def hessian_grid(lens: M.LensModel, grid: M.Grid, istart=0, jstart=0, precision=PREC_FP64, kmap=False):
+    planes = [np.empty(grid.npix) for _ in range(4)]
+    f = lib().lt_kmap_grid if kmap else lib().lt_gradient2_grid
+    check(f(*[M.dptr(p) for p in planes], C.byref(grid.frame), C.byref(lens.soa()), lens.n, grid.dx, grid.dy,
+            grid.nx, grid.ny, istart, jstart, precision))
+    return [p.reshape(grid.ny, grid.nx) for p in planes]
+
+
+def potential_grid(lens: M.LensModel, grid: M.Grid, istart=0, jstart=0):
+    phi = np.empty(grid.npix)
+    check(lib().lt_potential_grid(M.dptr(phi), C.byref(grid.frame), C.byref(lens.soa()), lens.n, grid.dx,
+                                  grid.dy, grid.nx, grid.ny, istart, jstart))
+    return phi.reshape(grid.ny, grid.nx)
+
+
+def map_grid(kind, mode, lens: M.LensModel, grid: M.Grid, cosmo, z_lens, z_source, istart=0, jstart=0,
+             precision=PREC_FP64):
+    m0 = np.empty(grid.npix)
+    m1 = np.empty(grid.npix) if kind == MAP_DPL else None
+    check(lib().lt_map_grid(kind, mode, M.dptr(m0), M.dptr(m1) if m1 is not None else None, C.byref(cosmo),
+                            C.byref(grid.frame), C.byref(lens.soa()), lens.n, z_lens, z_source, grid.dx,
+                            grid.dy, grid.nx, grid.ny, istart, jstart, precision))
+    s = (grid.ny, grid.nx)
+    return (m0.reshape(s), m1.reshape(s)) if m1 is not None else m0.reshape(s)
+
+
+def map_fields(lens: M.LensModel, grid: M.Grid, cosmo, z_lens, z_source, istart=0, jstart=0,
+               precision=PREC_FP64):
+    outs = [np.empty(grid.npix) for _ in range(5)]
+    check(lib().lt_map_fields(*[M.dptr(o) for o in outs], C.byref(cosmo), C.byref(grid.frame),
+                              C.byref(lens.soa()), lens.n, z_lens, z_source, grid.dx, grid.dy, grid.nx, grid.ny,
+                              istart, jstart, precision))
+    return [o.reshape(grid.ny, grid.nx) for o in outs]
+
+
+def cosmo_dlsds(zl, zs, cosmo):
+    return lib().lt_cosmo_dlsds(zl, zs, C.byref(cosmo))
+
+
+# ------------------------------------------------------------------------------------------------
+# device memory, streams, events
+# ------------------------------------------------------------------------------------------------
+
+class DeviceArray:
+    """A device allocation of `n` elements of `dtype` (owned; freed on close / garbage collection)."""
+
+    def __init__(self, n, dtype=np.float64):
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = self.n * self.dtype.itemsize
+        p = C.c_void_p()
+        check(lib().lt_dev_alloc(C.byref(p), self.nbytes))
+        self.ptr = p.value
+
+    def upload(self, host, stream=0):
+        host = np.ascontiguousarray(host, dtype=self.dtype)
+        assert host.nbytes <= self.nbytes
+        check(lib().lt_memcpy_h2d(C.c_void_p(self.ptr), host.ctypes.data_as(C.c_void_p), host.nbytes,
+                                  C.c_void_p(stream)))
+
+    def download(self, n=None, stream=0):
+        n = self.n if n is None else int(n)
+        out = np.empty(n, dtype=self.dtype)
+        check(lib().lt_memcpy_d2h(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), out.nbytes,
+                                  C.c_void_p(stream)))
+        if stream:
+            sync(stream)
+        return out
+
+    def fill_bytes(self, value: int, stream=0):
+        check(lib().lt_memset_d(C.c_void_p(self.ptr), int(value), self.nbytes, C.c_void_p(stream)))
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            lib().lt_dev_free(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def stream_create() -> int:
+    s = C.c_void_p()
+    check(lib().lt_stream_create(C.byref(s)))
+    return s.value
+
+
+def stream_destroy(s: int):
+    check(lib().lt_stream_destroy(C.c_void_p(s)))
+
+
+def sync(stream=0):
+    check(lib().lt_stream_sync(C.c_void_p(stream)))
+
+
+class Event:
+    def __init__(self):
+        e = C.c_void_p()
+        check(lib().lt_event_create(C.byref(e)))
+        self.h = e.value
+
+    def record(self, stream=0):
+        check(lib().lt_event_record(C.c_void_p(self.h), C.c_void_p(stream)))
+
+    def synchronize(self):
+        check(lib().lt_event_sync(C.c_void_p(self.h)))
+
+    def elapsed_ms(self, end: "Event") -> float:
+        ms = C.c_float(0.0)
+        check(lib().lt_event_elapsed(C.byref(ms), C.c_void_p(self.h), C.c_void_p(end.h)))
+        return ms.value
+
+    def __del__(self):
+        try:
+            lib().lt_event_destroy(C.c_void_p(self.h))
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------------------------------------
+# resident models (device pointers are plain integers, e.g. torch.Tensor.data_ptr())
+# ------------------------------------------------------------------------------------------------
+
+class Model:
+    def __init__(self, lens: M.LensModel, use=MAP_HESSIAN):
+        self.h = C.c_void_p()
+        self.lens = lens
+        check(lib().lt_model_create(C.byref(self.h), C.byref(lens.soa()), lens.n, use))
+
+    def update(self, lens: M.LensModel, stream=0):
+        self.lens = lens
+        check(lib().lt_model_update(self.h, C.byref(lens.soa()), lens.n, C.c_void_p(stream)))
+
+    def bytes(self) -> int:
+        b = C.c_longlong(0)
+        check(lib().lt_model_bytes(self.h, C.byref(b)))
+        return b.value
+
+    def map(self, kind, mode, outs, grid: M.Grid, r=1.0, mult=1.0, precision=PREC_FP64, px=0, stream=0,
+            istart=0, jstart=0, pitch=None):
+        g = LtGridDesc(grid.frame.xmin, grid.frame.ymin, grid.dx, grid.dy, grid.nx, grid.ny, istart, jstart,
+                       pitch or grid.nx)
+        ptrs = [C.c_void_p(p) if p else None for p in (list(outs) + [0] * 5)[:5]]
+        check(lib().lt_map_device(self.h, kind, mode, *ptrs, r, mult, C.byref(g), precision, px,
+                                  C.c_void_p(stream)))
+
+    def close(self):
+        if self.h:
+            lib().lt_model_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

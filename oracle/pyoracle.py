@@ -36,6 +36,7 @@ def lib():
         L.orc_point_gradient2.argtypes = [P(M.LtMatrix), P(M.LtPoint), soa, C.c_int]
         L.orc_point_kmap.argtypes = [P(M.LtMatrix), P(M.LtPoint), soa, C.c_int]
         L.orc_point_potential.argtypes = [dp, P(M.LtPoint), soa, C.c_int]
+        L.orc_point_gradient2_abs.argtypes = [P(M.LtMatrix), P(M.LtPoint), soa, C.c_int]
         L.orc_gradient_grid.argtypes = [dp, dp, gp, soa, C.c_int, C.c_double, C.c_double,
                                         C.c_int, C.c_int, C.c_int, C.c_int]
         L.orc_gradient_abs_planes.argtypes = [dp, dp, gp, soa, C.c_int, C.c_double, C.c_double,
@@ -130,6 +131,13 @@ def point_hessian(lens: M.LensModel, x: float, y: float, kmap=False, extended=Fa
     else:
         f = lib().orc_point_gradient2_ld if extended else lib().orc_point_gradient2
     _chk(f(C.byref(out), C.byref(p), C.byref(lens.soa()), lens.n), "hessian")
+    return out.a, out.b, out.c, out.d
+
+
+def point_hessian_abs(lens: M.LensModel, x: float, y: float):
+    p = M.LtPoint(x, y)
+    out = M.LtMatrix()
+    _chk(lib().orc_point_gradient2_abs(C.byref(out), C.byref(p), C.byref(lens.soa()), lens.n), "hessian abs")
     return out.a, out.b, out.c, out.d
 
 
