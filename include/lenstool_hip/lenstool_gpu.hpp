@@ -1,0 +1,134 @@
+// lenstool_gpu.hpp -- the reference's GPU call signatures (C++ linkage, default arguments and
+// overloads as in the reference headers), implemented in liblenstool_hip.so on top of the C-ABI
+// (lt_capi.h). A program written against the reference's grid_gradient_GPU.cuh,
+// grid_gradient2_GPU.cuh, grid_potential_GPU.cuh, grid_map_{ampli,shear,mass,pot,dpl}_GPU.cuh,
+// delense_GPU.cuh and chi_barycenter_GPU.cuh includes this header instead and links this library.
+//
+// Semantics kept: host arrays in and out, synchronous, same argument meaning and pixel convention.
+// Differences: device buffers are cached between calls instead of allocated and freed every time;
+// an unsupported halo type or a GPU error does not exit the process (the reference's cudasafe
+// calls exit(-1), src/cudafunctions.cu:33-39): the message goes to stderr and
+// lenstool_gpu_last_status() returns the LT_* code. Set LT_EXIT_ON_ERROR=1 to exit(-1) instead.
+#pragma once
+
+#include "structure_hpc.hpp"
+
+int lenstool_gpu_last_status();
+
+// ---- src/grid_gradient_GPU.cuh:34-38 ------------------------------------------------------------
+void gradient_grid_GPU(type_t* grid_grad_x, type_t* grid_grad_y, const struct grid_param* frame,
+                       const struct Potential_SOA* lens, int Nlens, int nbgridcells);
+void gradient_grid_GPU(type_t* grid_grad_x, type_t* grid_grad_y, const struct grid_param* frame,
+                       const struct Potential_SOA* lens, int nhalos, type_t dx, type_t dy, int nbgridcells_x,
+                       int nbgridcells_y, int istart = 0, int jstart = 0);
+void gradient_grid_GPU_UM(type_t* grid_grad_x, type_t* grid_grad_y, const struct grid_param* frame,
+                          const struct Potential_SOA* lens, int nhalos, type_t dx, type_t dy, int nbgridcells_x,
+                          int nbgridcells_y, int istart, int jstart);
+
+// ---- src/grid_gradient2_GPU.cuh:32-34 -----------------------------------------------------------
+void gradient2_grid_GPU(type_t* grid_grad2_a, type_t* grid_grad2_b, type_t* grid_grad2_c, type_t* grid_grad2_d,
+                        const struct grid_param* frame, const struct Potential_SOA* lens, int Nlens,
+                        int nbgridcells);
+void gradient2_grid_GPU(type_t* grid_grad2_a, type_t* grid_grad2_b, type_t* grid_grad2_c, type_t* grid_grad2_d,
+                        const struct grid_param* frame, const struct Potential_SOA* lens, int nhalos, type_t dx,
+                        type_t dy, int nbgridcells_x, int nbgridcells_y, int istart = 0, int jstart = 0);
+
+// ---- src/grid_potential_GPU.cuh:31-32 -----------------------------------------------------------
+void potential_grid_GPU(type_t* grid_potential, const struct grid_param* frame, const struct Potential_SOA* lens,
+                        int nhalos, type_t dx, type_t dy, int nbgridcells_x, int nbgridcells_y, int istart,
+                        int jstart);
+void potential_grid_GPU(type_t* grid_potential, const struct grid_param* frame, const struct Potential_SOA* lens,
+                        int nhalos, int nbgridcells);
+
+// ---- map functions: device planes in, device map out (src/grid_map_ampli_GPU.cuh:35-47 & co) ----
+typedef void (*map_gpu_function_t)(type_t* map, type_t* grid_grad2_a, type_t* grid_grad2_b, type_t* grid_grad2_c,
+                                   type_t* grid_grad2_d, type_t dl0s, type_t dos, type_t dl, type_t h, type_t z,
+                                   int nbgridcells_x, int nbgridcells_y, const struct grid_param* frame);
+typedef void (*map_pot_function_t)(type_t* map, type_t dl0s, type_t dos, type_t dl, type_t h, type_t z,
+                                   int nbgridcells_x, int nbgridcells_y, const struct grid_param* frame);
+
+map_gpu_function_t select_map_ampli_function(const struct runmode_param* runmode);
+map_gpu_function_t select_map_shear_function(const struct runmode_param* runmode);
+map_gpu_function_t select_map_mass_function(const struct runmode_param* runmode);
+map_pot_function_t select_map_potential_function(const struct runmode_param* runmode);
+
+void amplif_1_grid_CPU_GPU(type_t*, type_t*, type_t*, type_t*, type_t*, type_t, type_t, type_t, type_t, type_t, int,
+                           int, const struct grid_param*);
+void amplif_2_grid_CPU_GPU(type_t*, type_t*, type_t*, type_t*, type_t*, type_t, type_t, type_t, type_t, type_t, int,
+                           int, const struct grid_param*);
+void amplif_3_grid_CPU_GPU(type_t*, type_t*, type_t*, type_t*, type_t*, type_t, type_t, type_t, type_t, type_t, int,
+                           int, const struct grid_param*);
+void amplif_4_grid_CPU_GPU(type_t*, type_t*, type_t*, type_t*, type_t*, type_t, type_t, type_t, type_t, type_t, int,
+                           int, const struct grid_param*);
+void amplif_5_grid_CPU_GPU(type_t*, type_t*, type_t*, type_t*, type_t*, type_t, type_t, type_t, type_t, type_t, int,
+                           int, const struct grid_param*);
+void amplif_6_grid_CPU_GPU(type_t*, type_t*, type_t*, type_t*, type_t*, type_t, type_t, type_t, type_t, type_t, int,
+                           int, const struct grid_param*);
+void shear_1_grid_CPU_GPU(type_t*, type_t*, type_t*, type_t*, type_t*, type_t, type_t, type_t, type_t, type_t, int,
+                          int, const struct grid_param*);
+void shear_2_grid_CPU_GPU(type_t*, type_t*, type_t*, type_t*, type_t*, type_t, type_t, type_t, type_t, type_t, int,
+                          int, const struct grid_param*);
+void shear_3_grid_CPU_GPU(type_t*, type_t*, type_t*, type_t*, type_t*, type_t, type_t, type_t, type_t, type_t, int,
+                          int, const struct grid_param*);
+void shear_4_grid_CPU_GPU(type_t*, type_t*, type_t*, type_t*, type_t*, type_t, type_t, type_t, type_t, type_t, int,
+                          int, const struct grid_param*);
+void mass_1_grid_CPU_GPU(type_t*, type_t*, type_t*, type_t*, type_t*, type_t, type_t, type_t, type_t, type_t, int,
+                         int, const struct grid_param*);
+void mass_2_grid_CPU_GPU(type_t*, type_t*, type_t*, type_t*, type_t*, type_t, type_t, type_t, type_t, type_t, int,
+                         int, const struct grid_param*);
+void mass_3_grid_CPU_GPU(type_t*, type_t*, type_t*, type_t*, type_t*, type_t, type_t, type_t, type_t, type_t, int,
+                         int, const struct grid_param*);
+void mass_4_grid_CPU_GPU(type_t*, type_t*, type_t*, type_t*, type_t*, type_t, type_t, type_t, type_t, type_t, int,
+                         int, const struct grid_param*);
+void potential_1_grid_CPU_GPU(type_t*, type_t, type_t, type_t, type_t, type_t, int, int, const struct grid_param*);
+void potential_2_grid_CPU_GPU(type_t*, type_t, type_t, type_t, type_t, type_t, int, int, const struct grid_param*);
+void potential_3_grid_CPU_GPU(type_t*, type_t, type_t, type_t, type_t, type_t, int, int, const struct grid_param*);
+void dpl_grid_CPU_GPU(type_t* map, type_t dl0s, type_t ds, type_t dl, type_t h, type_t z, int nbgridcells_x,
+                      int nbgridcells_y, const struct grid_param* frame);
+
+// ---- whole maps, host in / host out (src/grid_map_*_GPU.cuh) -------------------------------------
+void map_grid_ampli_GPU(map_gpu_function_t mapfunction, type_t* amplif, const struct cosmo_param* cosmo,
+                        const struct grid_param* frame, const struct Potential_SOA* lens, int Nlens, int nbgridcells,
+                        int mode_amp, type_t z);
+void map_grid_ampli_GPU(map_gpu_function_t mapfunction, type_t* amplif, const struct cosmo_param* cosmo,
+                        const struct grid_param* frame, const struct Potential_SOA* lens, int Nlens, int mode_amp,
+                        type_t z, type_t dx, type_t dy, int nbgridcells_x, int nbgridcells_y, int istart = 0,
+                        int jstart = 0);
+void map_grid_shear_GPU(map_gpu_function_t mapfunction, type_t* amplif, const struct cosmo_param* cosmo,
+                        const struct grid_param* frame, const struct Potential_SOA* lens, int Nlens, int nbgridcells,
+                        int mode_amp, type_t z);
+void map_grid_shear_GPU(map_gpu_function_t mapfunction, type_t* amplif, const struct cosmo_param* cosmo,
+                        const struct grid_param* frame, const struct Potential_SOA* lens, int Nlens, int mode_amp,
+                        type_t z, type_t dx, type_t dy, int nbgridcells_x, int nbgridcells_y, int istart = 0,
+                        int jstart = 0);
+void map_grid_mass_GPU(map_gpu_function_t mapfunction, type_t* map, const struct cosmo_param* cosmo,
+                       const struct grid_param* frame, const struct Potential_SOA* lens, int nhalos, int nbgridcells,
+                       int mode_amp, type_t zm, type_t zs);
+void map_grid_mass_GPU(map_gpu_function_t mapfunction, type_t* map, const struct cosmo_param* cosmo,
+                       const struct grid_param* frame, const struct Potential_SOA* lens, int nhalos, int mode_amp,
+                       type_t zm, type_t zs, type_t dx, type_t dy, int nbgridcells_x, int nbgridcells_y, int istart,
+                       int jstart);
+void map_grid_potential_GPU(map_pot_function_t mapfunction, type_t* amplif, const struct cosmo_param* cosmo,
+                            const struct grid_param* frame, const struct Potential_SOA* lens, int Nlens,
+                            int nbgridcells, int mode_amp, type_t z);
+void map_grid_potential_GPU(map_pot_function_t mapfunction, type_t* amplif, const struct cosmo_param* cosmo,
+                            const struct grid_param* frame, const struct Potential_SOA* lens, int Nlens, int mode_amp,
+                            type_t z, type_t dx, type_t dy, int nbgridcells_x, int nbgridcells_y, int istart = 0,
+                            int jstart = 0);
+void map_grid_dpl_GPU(map_gpu_function_t mapfunction, type_t* dplx, type_t* dply, const struct cosmo_param* cosmo,
+                      const struct grid_param* frame, const struct Potential_SOA* lens, int Nlens, int nbgridcells,
+                      int mode_amp, type_t z);
+void map_grid_dpl_GPU(map_gpu_function_t mapfunction, type_t* dplx, type_t* dply, const struct cosmo_param* cosmo,
+                      const struct grid_param* frame, const struct Potential_SOA* lens, int Nlens, int mode_amp,
+                      type_t z, type_t dx, type_t dy, int nbgridcells_x, int nbgridcells_y, int istart = 0,
+                      int jstart = 0);
+
+// ---- delensing and chi^2 (src/delense_GPU.cuh:27, src/chi_barycenter_GPU.cuh:13) ----------------
+void delense_barycenter_GPU(struct point* image_pos, int* locimagesfound, int* numimg, const runmode_param* runmode,
+                            const struct Potential_SOA* lens, const struct grid_param* frame,
+                            const int* nimages_strongLensing, const struct galaxy* sources, double* grid_gradient_x,
+                            double* grid_gradient_y);
+void mychi_bruteforce_SOA_GPU_grid_gradient_barycentersource(double* chi, int* error, runmode_param* runmode,
+                                                             const struct Potential_SOA* lens,
+                                                             const struct grid_param* frame,
+                                                             const int* nimages_strongLensing, galaxy* images);

@@ -135,6 +135,63 @@ int lt_map_device(lt_model model, int kind, int mode, double* out0, double* out1
                   double* out4, double r, double mult, const lt_grid_desc* grid, int precision, int px,
                   void* stream);
 
+/* ---- point evaluations, delensing and chi^2 ------------------------------------------------------ */
+
+/* Gradient at npts points (host arrays). The batched counterpart of the reference's point evaluator
+ * module_potentialDerivatives_totalGradient_SOA (src/gradient.cpp:899-930). */
+int lt_point_gradients(double* grad_x, double* grad_y, const double* px, const double* py, int npts,
+                       const lt_potential_soa* lens, int nhalos);
+
+/* Image-plane triangles of an nbgridcells^2 gradient grid mapped to the source plane, images found
+ * for each source. Replaces delense_barycenter_GPU (src/delense_GPU.cuh:27,
+ * src/delense_GPU.cu:227-350): image_pos has nsets*LT_MAXIMPERSOURCE slots, locimagesfound nsets
+ * entries, numimg is incremented by the number of hits. Images come in the reference CPU path's
+ * single-thread scan order (row by row, upper triangle before lower). A source with more than
+ * LT_MAXIMPERSOURCE images keeps the first ones and the call returns LT_ERR_OVERFLOW (the
+ * reference writes past its buffer). gx, gy: host gradient planes, sources[k].center and .dr used. */
+int lt_delense_barycenter(lt_point* image_pos, int* locimagesfound, int* numimg, int nsets, int nbgridcells,
+                          const lt_grid_param* frame, const lt_galaxy* sources, const double* gx,
+                          const double* gy);
+
+/* chi^2 of one lens model against nsets families of observed images (nimages[k] each, images in
+ * family order, .center and .dr used): gradient grid, source barycentres, delensing and matching as
+ * mychi_bruteforce_SOA_GPU_grid_gradient_barycentersource (src/chi_barycenter_GPU.cuh:13,
+ * src/chi_CPU_barycenter.cpp:58-287) on one rank; *error receives the status. */
+int lt_chi_barycentersource(double* chi, int* error, int nbgridcells, int nsets, int nhalos,
+                            const lt_potential_soa* lens, const lt_grid_param* frame, const int* nimages,
+                            const lt_galaxy* images);
+
+/* chi^2 of found images against observed families, the matching and sum of chi_computation
+ * (src/chi_computation.hpp:3-4, src/chi_computation.cpp:10-195) on the GPU: *chi and *images_found are
+ * incremented as the reference does. imagesposition has nsets*LT_MAXIMPERSOURCE slots. */
+int lt_chi_computation(double* chi, int* images_found, const int* numimagesfound, const lt_point* imagesposition,
+                       const int* nimages, const lt_galaxy* images, int nsets);
+
+/* The same for a batch of lens-model samples sharing halo types (e.g. MCMC samples): created once
+ * for the observed families and the delensing grid, then run for nsamples SOA models at a time on
+ * `stream`. chi, found (images found) and status are host arrays of nsamples; the call returns when
+ * they are filled. No reference counterpart (the reference loops over samples on the host). */
+typedef struct lt_chi_batch_s* lt_chi_batch;
+int lt_chi_batch_create(lt_chi_batch* out, int nsamples, int nhalos, const int* types, int nbgridcells,
+                        const lt_grid_param* frame, int nsets, const int* nimages, const lt_galaxy* images);
+int lt_chi_batch_run(lt_chi_batch batch, const lt_potential_soa* lenses, double* chi, int* found, int* status,
+                     void* stream);
+/* images found for one sample of the last run (nsets*LT_MAXIMPERSOURCE slots, zero past the count) */
+int lt_chi_batch_images(lt_chi_batch batch, int sample, lt_point* image_pos, int* locimagesfound);
+int lt_chi_batch_destroy(lt_chi_batch batch);
+
+/* x[i] *= s for n doubles in device memory (the reference's in-place potential and displacement
+ * scalings, src/grid_map_pot_GPU.cu:201-250, src/grid_map_dpl_GPU.cu:194-208). */
+int lt_scale_device(double* x, double s, long long n, void* stream);
+
+/* The map epilogue alone, on Hessian planes a, b, c already in device memory (n pixels), for kinds
+ * LT_MAP_AMPLI, LT_MAP_SHEAR, LT_MAP_MASS (one output plane out0) and LT_MAP_FIELDS (out0..out4).
+ * This is what the reference's map functions (amplif_k_grid_CPU_GPU, shear_k..., mass_k...,
+ * src/grid_map_ampli_GPU.cu:305-547) do after their separate Hessian launch. */
+int lt_map_epilogue_device(int kind, int mode, double* out0, double* out1, double* out2, double* out3,
+                           double* out4, const double* a, const double* b, const double* c, long long n,
+                           double r, double mult, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

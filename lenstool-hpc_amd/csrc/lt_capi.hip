@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "lenstool_hip/lt_capi.h"
+#include "lt_internal.h"
 #include "lt_kernels.h"
 
 using namespace lt;
@@ -18,23 +19,43 @@ namespace {
 thread_local std::string g_err;
 thread_local int g_device = -1;
 
-int fail(int code, const std::string& msg)
+}  // namespace
+
+namespace lt {
+
+int set_error(int code, const std::string& msg)
 {
     g_err = msg;
     return code;
 }
 
-int hip_fail(hipError_t e, const char* where)
+int hip_error(hipError_t e, const char* where)
 {
     std::string m = std::string(where) + ": " + hipGetErrorString(e);
-    return fail(LT_ERR_HIP, m);
+    return set_error(LT_ERR_HIP, m);
 }
 
-#define HIP_TRY(expr)                                         \
-    do {                                                      \
-        hipError_t e_ = (expr);                               \
-        if (e_ != hipSuccess) return hip_fail(e_, #expr);     \
-    } while (0)
+int current_device(int* dev)
+{
+    if (g_device < 0) {
+        int n = 0;
+        hipError_t e = hipGetDeviceCount(&n);
+        if (e != hipSuccess || n <= 0) return set_error(LT_ERR_NO_DEVICE, "no GPU visible to the HIP runtime");
+        LT_HIP_TRY(hipGetDevice(&g_device));
+    }
+    if (g_device >= 64) return set_error(LT_ERR_NO_DEVICE, "device index out of range");
+    LT_HIP_TRY(hipSetDevice(g_device));
+    *dev = g_device;
+    return LT_OK;
+}
+
+}  // namespace lt
+
+namespace {
+
+int fail(int code, const std::string& msg) { return set_error(code, msg); }
+int hip_fail(hipError_t e, const char* where) { return hip_error(e, where); }
+#define HIP_TRY(expr) LT_HIP_TRY(expr)
 
 // Per-device state of the synchronous host-array entry points: a stream and grow-only buffers.
 struct DeviceState {
@@ -48,20 +69,6 @@ struct DeviceState {
 
 constexpr int kMaxDevices = 64;
 DeviceState g_state[kMaxDevices];
-
-int current_device(int* dev)
-{
-    if (g_device < 0) {
-        int n = 0;
-        hipError_t e = hipGetDeviceCount(&n);
-        if (e != hipSuccess || n <= 0) return fail(LT_ERR_NO_DEVICE, "no GPU visible to the HIP runtime");
-        HIP_TRY(hipGetDevice(&g_device));
-    }
-    if (g_device >= kMaxDevices) return fail(LT_ERR_NO_DEVICE, "device index out of range");
-    HIP_TRY(hipSetDevice(g_device));
-    *dev = g_device;
-    return LT_OK;
-}
 
 int ensure(void** buf, size_t* have, size_t need)
 {
@@ -225,6 +232,7 @@ int host_map(int kind, int mode, Use use, double* const host_out[5], const lt_gr
     g.istart = istart;
     g.jstart = jstart;
     g.pitch = nx;
+    g.nsamples = 1;
     if ((st = launch_map(m, kind, mode, dev_out, r, mult, g, precision, 0, S.stream))) return st;
     for (int k = 0; k < np; ++k)
         if (host_out[k])
@@ -634,8 +642,50 @@ int lt_map_device(lt_model m, int kind, int mode, double* out0, double* out1, do
     g.istart = grid->istart;
     g.jstart = grid->jstart;
     g.pitch = grid->pitch;
+    g.nsamples = 1;
     HIP_TRY(hipSetDevice(m->device));
     return launch_map(m->dm, kind, mode, outs, r, mult, g, precision, px, (hipStream_t)stream);
+}
+
+int lt_scale_device(double* x, double s, long long n, void* stream)
+{
+    if (n < 0 || (n && !x)) return fail(LT_ERR_INVALID, "bad array");
+    if (!n) return LT_OK;
+    int dev, st;
+    if ((st = current_device(&dev))) return st;
+    hipError_t e = launch_scale(x, s, (size_t)n, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "scale launch");
+    if (!stream) HIP_TRY(hipStreamSynchronize(nullptr));
+    return LT_OK;
+}
+
+int lt_map_epilogue_device(int kind, int mode, double* out0, double* out1, double* out2, double* out3, double* out4,
+                           const double* a, const double* b, const double* c, long long n, double r, double mult,
+                           void* stream)
+{
+    int st = check_kind_mode(kind, mode);
+    if (st) return st;
+    if (kind != LT_MAP_AMPLI && kind != LT_MAP_SHEAR && kind != LT_MAP_MASS && kind != LT_MAP_FIELDS)
+        return fail(LT_ERR_INVALID, "epilogue kinds are AMPLI, SHEAR, MASS and FIELDS");
+    if (n < 0 || (n && (!a || !b || !c))) return fail(LT_ERR_INVALID, "bad Hessian planes");
+    if (kind != LT_MAP_FIELDS && !out0) return fail(LT_ERR_INVALID, "out0 is NULL");
+    if (!n) return LT_OK;
+    int dev;
+    if ((st = current_device(&dev))) return st;
+    OutDesc o{};
+    o.p0 = out0;
+    o.p1 = out1;
+    o.p2 = out2;
+    o.p3 = out3;
+    o.p4 = out4;
+    o.kind = kind;
+    o.mode = mode;
+    o.r = r;
+    o.mult = mult;
+    hipError_t e = launch_epilogue64(a, b, c, (int64_t)n, o, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "epilogue launch");
+    if (!stream) HIP_TRY(hipStreamSynchronize(nullptr));
+    return LT_OK;
 }
 
 }  // extern "C"

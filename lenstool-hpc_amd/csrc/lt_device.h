@@ -68,6 +68,12 @@ struct GridDesc {
     double xmin, ymin, dx, dy;
     int32_t nx, ny, istart, jstart;
     int64_t pitch;
+    // batches of lens models on one grid (blockIdx.z = sample): sample s reads halo records
+    // [s*halo_stride, s*halo_stride + nhalos) and writes its planes at offset s*plane_stride
+    int64_t halo_stride;
+    int64_t plane_stride;
+    int32_t nsamples;
+    int32_t pad;
 };
 
 // What the epilogue writes. kind / mode follow lt_status.h (LT_MAP_*).

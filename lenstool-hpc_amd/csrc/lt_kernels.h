@@ -18,6 +18,14 @@ hipError_t launch_gradient64(const HaloRec* halos, const Run* runs, int nruns, c
                              const OutDesc& o, int px, hipStream_t s);
 hipError_t launch_potential64(const HaloRec* halos, const Run* runs, int nruns, const GridDesc& g,
                               const OutDesc& o, hipStream_t s);
+// gradient at npts points for nsamples lens models (records of sample s start at s*halo_stride);
+// outputs [nsamples*npts]
+hipError_t launch_points_gradient64(const HaloRec* halos, const Run* runs, int nruns, const double* px,
+                                   const double* py, int npts, int nsamples, int64_t halo_stride, double* gx,
+                                   double* gy, hipStream_t s);
+// map epilogue alone on device Hessian planes (kinds FIELDS, AMPLI, SHEAR, MASS)
+hipError_t launch_epilogue64(const double* a, const double* b, const double* c, int64_t n, const OutDesc& o,
+                             hipStream_t s);
 
 // FP32-compensated kernels (lt_kernels32.hip).
 hipError_t launch_hessian32(const HaloRec* halos64, const HaloRec32* halos32, const Run* runs, int nruns,
@@ -27,6 +35,8 @@ hipError_t launch_gradient32(const HaloRec* halos64, const HaloRec32* halos32, c
 
 // Streaming device copy (16-byte vectors plus a byte tail).
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
+// x *= s on n doubles
+hipError_t launch_scale(double* x, double s, size_t n, hipStream_t st);
 
 // What a packed model is used for: the run split differs for the Kmap (runs also break on dlsds).
 enum class Use { Gradient, Hessian, Kmap, Potential };

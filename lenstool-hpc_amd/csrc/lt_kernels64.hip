@@ -305,6 +305,19 @@ __device__ __forceinline__ void hess_epilogue(const OutDesc& o, int64_t idx, dou
 constexpr int WAVE = 64;
 constexpr int ROWS = 4;
 
+// Sample s of a batch (blockIdx.z) uses its own halo records and output planes.
+__device__ __forceinline__ const HaloRec* select_sample(const GridDesc& g, const HaloRec* halos, OutDesc& o)
+{
+    const int64_t s = blockIdx.z;
+    const int64_t off = s * g.plane_stride;
+    if (o.p0) o.p0 += off;
+    if (o.p1) o.p1 += off;
+    if (o.p2) o.p2 += off;
+    if (o.p3) o.p3 += off;
+    if (o.p4) o.p4 += off;
+    return halos + s * g.halo_stride;
+}
+
 template <int PX, int KIND>
 __global__ __launch_bounds__(WAVE* ROWS) void k_hessian64(const HaloRec* __restrict__ halos,
                                                           const Run* __restrict__ runs, int nruns,
@@ -312,6 +325,7 @@ __global__ __launch_bounds__(WAVE* ROWS) void k_hessian64(const HaloRec* __restr
 {
     const int row = blockIdx.y * ROWS + threadIdx.y;
     if (row >= g.ny) return;
+    halos = select_sample(g, halos, o);
     const int col0 = blockIdx.x * (WAVE * PX) + threadIdx.x;
     const double py = pixel_coord(g.ymin, row + g.jstart, g.dy);
     double px[PX];
@@ -382,6 +396,7 @@ __global__ __launch_bounds__(WAVE* ROWS) void k_gradient64(const HaloRec* __rest
 {
     const int row = blockIdx.y * ROWS + threadIdx.y;
     if (row >= g.ny) return;
+    halos = select_sample(g, halos, o);
     const int col0 = blockIdx.x * (WAVE * PX) + threadIdx.x;
     const double py = pixel_coord(g.ymin, row + g.jstart, g.dy);
     double px[PX], tx[PX], ty[PX];
@@ -444,6 +459,7 @@ __global__ __launch_bounds__(WAVE* ROWS) void k_potential64(const HaloRec* __res
     const int row = blockIdx.y * ROWS + threadIdx.y;
     const int col = blockIdx.x * WAVE + threadIdx.x;
     if (row >= g.ny || col >= g.nx) return;
+    halos = select_sample(g, halos, o);
     const double px = pixel_coord(g.xmin, col + g.istart, g.dx);
     const double py = pixel_coord(g.ymin, row + g.jstart, g.dy);
     double tot = 0.;
@@ -456,12 +472,63 @@ __global__ __launch_bounds__(WAVE* ROWS) void k_potential64(const HaloRec* __res
     o.p0[(int64_t)row * g.pitch + col] = tot * o.mult;
 }
 
+// Gradient at arbitrary points for a batch of lens models (blockIdx.y = sample), e.g. at the
+// observed images for the source barycentres. Same per-run accumulation as the grid kernel.
+__global__ __launch_bounds__(256) void k_points_gradient64(const HaloRec* __restrict__ halos,
+                                                           const Run* __restrict__ runs, int nruns,
+                                                           const double* __restrict__ px,
+                                                           const double* __restrict__ py, int npts,
+                                                           int64_t halo_stride, double* __restrict__ gx,
+                                                           double* __restrict__ gy)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t s = blockIdx.y;
+    if (i >= npts) return;
+    halos += s * halo_stride;
+    const double x = px[i], y = py[i];
+    double tx = 0., ty = 0.;
+    for (int r = 0; r < nruns; ++r) {
+        const Run run = runs[r];
+        double sx = 0., sy = 0.;
+        const int end = run.start + run.count;
+        switch (run.type) {
+        case TYPE_DPIE:
+            for (int h = run.start; h < end; ++h) grad_piemd<true>(halos[h], x, y, sx, sy);
+            break;
+        case TYPE_PIEMD:
+            for (int h = run.start; h < end; ++h) grad_piemd<false>(halos[h], x, y, sx, sy);
+            break;
+        default:
+            for (int h = run.start; h < end; ++h) grad_sis(halos[h], x, y, sx, sy);
+            break;
+        }
+        tx += sx;
+        ty += sy;
+    }
+    gx[s * npts + i] = tx;
+    gy[s * npts + i] = ty;
+}
+
+// Epilogue alone, on Hessian planes already in device memory (the reference's map functions
+// amplif_k_grid_CPU_GPU & co. work this way, src/grid_map_ampli_GPU.cu:305-547).
+template <int KIND>
+__global__ __launch_bounds__(256) void k_epilogue(const double* __restrict__ a, const double* __restrict__ b,
+                                                  const double* __restrict__ c, int64_t n, OutDesc o)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const double av = a[i], cv = c[i];
+        hess_epilogue<KIND>(o, i, av + cv, av - cv, b[i]);
+    }
+}
+
 // ------------------------------------------------------------------------------------------------
 // launchers
 // ------------------------------------------------------------------------------------------------
 static dim3 grid_for(const GridDesc& g, int px)
 {
-    return dim3((unsigned)((g.nx + WAVE * px - 1) / (WAVE * px)), (unsigned)((g.ny + ROWS - 1) / ROWS), 1);
+    return dim3((unsigned)((g.nx + WAVE * px - 1) / (WAVE * px)), (unsigned)((g.ny + ROWS - 1) / ROWS),
+                (unsigned)(g.nsamples > 0 ? g.nsamples : 1));
 }
 
 template <int PX>
@@ -487,6 +554,31 @@ hipError_t launch_hessian64(const HaloRec* halos, const Run* runs, int nruns, co
     case 4: return launch_hessian64_px<4>(halos, runs, nruns, g, o, s);
     default: return launch_hessian64_px<2>(halos, runs, nruns, g, o, s);
     }
+}
+
+hipError_t launch_epilogue64(const double* a, const double* b, const double* c, int64_t n, const OutDesc& o,
+                             hipStream_t s)
+{
+    int64_t nb = (n + 255) / 256;
+    unsigned blocks = (unsigned)(nb > 16384 ? 16384 : (nb ? nb : 1));
+    dim3 blk(256), grd(blocks);
+    switch (o.kind) {
+    case LT_MAP_FIELDS: hipLaunchKernelGGL((k_epilogue<LT_MAP_FIELDS>), grd, blk, 0, s, a, b, c, n, o); break;
+    case LT_MAP_AMPLI: hipLaunchKernelGGL((k_epilogue<LT_MAP_AMPLI>), grd, blk, 0, s, a, b, c, n, o); break;
+    case LT_MAP_SHEAR: hipLaunchKernelGGL((k_epilogue<LT_MAP_SHEAR>), grd, blk, 0, s, a, b, c, n, o); break;
+    case LT_MAP_MASS: hipLaunchKernelGGL((k_epilogue<LT_MAP_MASS>), grd, blk, 0, s, a, b, c, n, o); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_points_gradient64(const HaloRec* halos, const Run* runs, int nruns, const double* px,
+                                   const double* py, int npts, int nsamples, int64_t halo_stride, double* gx,
+                                   double* gy, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_points_gradient64, dim3((npts + 255) / 256, nsamples), dim3(256), 0, s, halos, runs, nruns,
+                       px, py, npts, halo_stride, gx, gy);
+    return hipGetLastError();
 }
 
 hipError_t launch_gradient64(const HaloRec* halos, const Run* runs, int nruns, const GridDesc& g,

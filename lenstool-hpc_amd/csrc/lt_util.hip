@@ -1,6 +1,7 @@
 // lt_util.hip -- small utility kernels.
 #include <hip/hip_runtime.h>
 
+#include "lt_internal.h"
 #include "lt_kernels.h"
 
 namespace lt {
@@ -37,6 +38,31 @@ hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s)
     if (bytes > body)
         hipLaunchKernelGGL(k_copy1, dim3(blocks_for(bytes - body)), dim3(256), 0, s, (unsigned char*)dst + body,
                            (const unsigned char*)src + body, bytes - body);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_scale(double* __restrict__ x, double s, size_t n)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) x[i] *= s;
+}
+
+hipError_t launch_scale(double* x, double s, size_t n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_scale, dim3(blocks_for(n)), dim3(256), 0, st, x, s, n);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_fill_i32(int32_t* __restrict__ p, int32_t v, size_t n)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+hipError_t launch_fill_i32(int32_t* p, int32_t v, size_t n, hipStream_t s)
+{
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(n)), dim3(256), 0, s, p, v, n);
     return hipGetLastError();
 }
 

@@ -83,6 +83,22 @@ SIGNATURES = {
     "lt_map_device": ([C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                        C.c_void_p, C.c_double, C.c_double, C.POINTER(LtGridDesc), C.c_int, C.c_int,
                        C.c_void_p], C.c_int),
+    "lt_point_gradients": ([_dp, _dp, _dp, _dp, C.c_int, _soa, C.c_int], C.c_int),
+    "lt_delense_barycenter": ([C.POINTER(M.LtPoint), C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, C.c_int, _gp,
+                               C.POINTER(M.LtGalaxy), _dp, _dp], C.c_int),
+    "lt_chi_barycentersource": ([_dp, C.POINTER(C.c_int), C.c_int, C.c_int, C.c_int, _soa, _gp, C.POINTER(C.c_int),
+                                 C.POINTER(M.LtGalaxy)], C.c_int),
+    "lt_chi_computation": ([_dp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(M.LtPoint), C.POINTER(C.c_int),
+                            C.POINTER(M.LtGalaxy), C.c_int], C.c_int),
+    "lt_chi_batch_create": ([C.POINTER(C.c_void_p), C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int, _gp, C.c_int,
+                             C.POINTER(C.c_int), C.POINTER(M.LtGalaxy)], C.c_int),
+    "lt_chi_batch_run": ([C.c_void_p, _soa, _dp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_void_p], C.c_int),
+    "lt_chi_batch_images": ([C.c_void_p, C.c_int, C.POINTER(M.LtPoint), C.POINTER(C.c_int)], C.c_int),
+    "lt_chi_batch_destroy": ([C.c_void_p], C.c_int),
+    "lt_scale_device": ([C.c_void_p, C.c_double, C.c_longlong, C.c_void_p], C.c_int),
+    "lt_map_epilogue_device": ([C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                C.c_void_p, C.c_void_p, C.c_void_p, C.c_longlong, C.c_double, C.c_double,
+                                C.c_void_p], C.c_int),
 }
 
 
@@ -159,6 +175,96 @@ def map_fields(lens: M.LensModel, grid: M.Grid, cosmo, z_lens, z_source, istart=
                               C.byref(lens.soa()), lens.n, z_lens, z_source, grid.dx, grid.dy, grid.nx, grid.ny,
                               istart, jstart, precision))
     return [o.reshape(grid.ny, grid.nx) for o in outs]
+
+
+def point_gradients(lens: M.LensModel, px, py):
+    px = np.ascontiguousarray(px, dtype=np.float64)
+    py = np.ascontiguousarray(py, dtype=np.float64)
+    gx = np.empty_like(px)
+    gy = np.empty_like(py)
+    check(lib().lt_point_gradients(M.dptr(gx), M.dptr(gy), M.dptr(px), M.dptr(py), px.size, C.byref(lens.soa()),
+                                   lens.n))
+    return gx, gy
+
+
+def delense_barycenter(nsets, n, frame, sources: np.ndarray, gx, gy):
+    """Returns (status, image_pos[nsets, 20, 2], locimagesfound[nsets], numimg)."""
+    pos = np.zeros((nsets, M.MAXIMPERSOURCE, 2))
+    loc = np.zeros(nsets, dtype=np.int32)
+    numimg = C.c_int(0)
+    gx = np.ascontiguousarray(gx, dtype=np.float64).ravel()
+    gy = np.ascontiguousarray(gy, dtype=np.float64).ravel()
+    st = lib().lt_delense_barycenter(pos.ctypes.data_as(C.POINTER(M.LtPoint)), loc.ctypes.data_as(C.POINTER(C.c_int)),
+                                     C.byref(numimg), nsets, n, C.byref(frame), M.galaxy_ptr(sources), M.dptr(gx),
+                                     M.dptr(gy))
+    if st not in (LT_OK, 5):
+        check(st)
+    return st, pos, loc, numimg.value
+
+
+def chi_barycentersource(lens: M.LensModel, n, frame, nimages, images):
+    nimages = np.ascontiguousarray(nimages, dtype=np.int32)
+    chi = C.c_double(0.0)
+    err = C.c_int(0)
+    st = lib().lt_chi_barycentersource(C.byref(chi), C.byref(err), n, len(nimages), lens.n, C.byref(lens.soa()),
+                                       C.byref(frame), nimages.ctypes.data_as(C.POINTER(C.c_int)),
+                                       M.galaxy_ptr(images))
+    return st, chi.value
+
+
+def chi_computation(loc, pos, nimages, images):
+    nimages = np.ascontiguousarray(nimages, dtype=np.int32)
+    loc = np.ascontiguousarray(loc, dtype=np.int32)
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    chi = C.c_double(0.0)
+    found = C.c_int(0)
+    check(lib().lt_chi_computation(C.byref(chi), C.byref(found), loc.ctypes.data_as(C.POINTER(C.c_int)),
+                                   pos.ctypes.data_as(C.POINTER(M.LtPoint)), nimages.ctypes.data_as(C.POINTER(C.c_int)),
+                                   M.galaxy_ptr(images), len(nimages)))
+    return chi.value, found.value
+
+
+class ChiBatch:
+    """chi^2 of many lens-model samples (same halo types) against one set of image families."""
+
+    def __init__(self, nsamples, types, n, frame, nimages, images):
+        self.h = C.c_void_p()
+        self.nsamples = int(nsamples)
+        self.nsets = len(nimages)
+        self._keep = (np.ascontiguousarray(types, dtype=np.int32), np.ascontiguousarray(nimages, dtype=np.int32),
+                      np.ascontiguousarray(images))
+        t, ni, im = self._keep
+        check(lib().lt_chi_batch_create(C.byref(self.h), self.nsamples, t.size, t.ctypes.data_as(C.POINTER(C.c_int)),
+                                        n, C.byref(frame), ni.size, ni.ctypes.data_as(C.POINTER(C.c_int)),
+                                        M.galaxy_ptr(im)))
+
+    def run(self, lenses, stream=0):
+        assert len(lenses) == self.nsamples
+        arr = (M.LtPotentialSOA * self.nsamples)(*[l.soa() for l in lenses])
+        chi = np.empty(self.nsamples)
+        found = np.empty(self.nsamples, dtype=np.int32)
+        status = np.empty(self.nsamples, dtype=np.int32)
+        check(lib().lt_chi_batch_run(self.h, arr, M.dptr(chi), found.ctypes.data_as(C.POINTER(C.c_int)),
+                                     status.ctypes.data_as(C.POINTER(C.c_int)), C.c_void_p(stream)))
+        return chi, found, status
+
+    def images(self, sample):
+        pos = np.zeros((self.nsets, M.MAXIMPERSOURCE, 2))
+        loc = np.zeros(self.nsets, dtype=np.int32)
+        check(lib().lt_chi_batch_images(self.h, sample, pos.ctypes.data_as(C.POINTER(M.LtPoint)),
+                                        loc.ctypes.data_as(C.POINTER(C.c_int))))
+        return pos, loc
+
+    def close(self):
+        if self.h:
+            lib().lt_chi_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def cosmo_dlsds(zl, zs, cosmo):

@@ -1,0 +1,109 @@
+// shim_check.cpp -- drives the reference-signature entry points (lenstool_gpu.hpp) the way a
+// reference caller would: reference structures, host arrays, map-function selectors.
+//   shim_check <input.bin> <output.bin>
+// input : int n, nx, ny; double xmin, xmax, ymin, ymax, zs; int type[n];
+//         double x, y, b0, angle, epot, rcore, rcut, z, dlsds [n each]
+// output: doubles, in order: gx, gy (square overload), a, b, c, d (dx/dy overload, istart=1, jstart=2),
+//         ampli mode 1, ampli mode 1 through a foreign map function, shear mode 3, mass mode 2,
+//         potential mode 1, dplx, dply; then the int status of lenstool_gpu_last_status().
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "lenstool_hip/lenstool_gpu.hpp"
+
+static void foreign_ampli1(type_t* map, type_t* a, type_t* b, type_t* c, type_t* d, type_t dl0s, type_t dos, type_t dl,
+                           type_t h, type_t z, int nx, int ny, const grid_param* frame)
+{
+    amplif_1_grid_CPU_GPU(map, a, b, c, d, dl0s, dos, dl, h, z, nx, ny, frame);
+}
+
+int main(int argc, char** argv)
+{
+    if (argc != 3) return 2;
+    FILE* f = std::fopen(argv[1], "rb");
+    if (!f) return 3;
+    int n, nx, ny;
+    double xmin, xmax, ymin, ymax, zs;
+    if (std::fread(&n, 4, 1, f) != 1 || std::fread(&nx, 4, 1, f) != 1 || std::fread(&ny, 4, 1, f) != 1) return 4;
+    if (std::fread(&xmin, 8, 1, f) != 1 || std::fread(&xmax, 8, 1, f) != 1 || std::fread(&ymin, 8, 1, f) != 1 ||
+        std::fread(&ymax, 8, 1, f) != 1 || std::fread(&zs, 8, 1, f) != 1)
+        return 4;
+    std::vector<int> type(n);
+    std::vector<double> arr[9];
+    if (std::fread(type.data(), 4, n, f) != (size_t)n) return 4;
+    for (auto& a : arr) {
+        a.resize(n);
+        if (std::fread(a.data(), 8, n, f) != (size_t)n) return 4;
+    }
+    std::fclose(f);
+    std::vector<double> cs(n), sn(n);
+    for (int i = 0; i < n; ++i) {
+        cs[i] = std::cos(arr[3][i]);
+        sn[i] = std::sin(arr[3][i]);
+    }
+    Potential_SOA lens{};
+    lens.type = type.data();
+    lens.position_x = arr[0].data();
+    lens.position_y = arr[1].data();
+    lens.b0 = arr[2].data();
+    lens.ellipticity_angle = arr[3].data();
+    lens.ellipticity_potential = arr[4].data();
+    lens.rcore = arr[5].data();
+    lens.rcut = arr[6].data();
+    lens.z = arr[7].data();
+    lens.dlsds = arr[8].data();
+    lens.anglecos = cs.data();
+    lens.anglesin = sn.data();
+    grid_param frame{xmin, xmax, ymin, ymax, 0, 0, 0};
+    cosmo_param cosmo{1, 0.3, 0.7, 0.0, -1.0, 0.0, 70.0, 1.4};
+    runmode_param rm{};
+    const size_t N = (size_t)nx * ny;
+    std::vector<double> out;
+    std::vector<double> p0(N), p1(N), p2(N), p3(N);
+    auto put = [&](const std::vector<double>& v) { out.insert(out.end(), v.begin(), v.end()); };
+    int worst = 0;
+    auto st = [&]() { int s = lenstool_gpu_last_status(); if (s) worst = s; };
+
+    gradient_grid_GPU(p0.data(), p1.data(), &frame, &lens, n, nx);  // square overload (nx == ny)
+    st();
+    put(p0);
+    put(p1);
+    const double dx = (xmax - xmin) / (nx - 1), dy = (ymax - ymin) / (ny - 1);
+    gradient2_grid_GPU(p0.data(), p1.data(), p2.data(), p3.data(), &frame, &lens, n, dx, dy, nx, ny, 1, 2);
+    st();
+    put(p0);
+    put(p1);
+    put(p2);
+    put(p3);
+    rm.amplif = 1;
+    map_grid_ampli_GPU(select_map_ampli_function(&rm), p0.data(), &cosmo, &frame, &lens, n, nx, 1, zs);
+    st();
+    put(p0);
+    map_grid_ampli_GPU(foreign_ampli1, p0.data(), &cosmo, &frame, &lens, n, nx, 1, zs);
+    st();
+    put(p0);
+    rm.shear = 3;
+    map_grid_shear_GPU(select_map_shear_function(&rm), p0.data(), &cosmo, &frame, &lens, n, nx, 3, zs);
+    st();
+    put(p0);
+    rm.mass = 2;
+    map_grid_mass_GPU(select_map_mass_function(&rm), p0.data(), &cosmo, &frame, &lens, n, nx, 2, 0.0, zs);
+    st();
+    put(p0);
+    rm.potential = 1;
+    map_grid_potential_GPU(select_map_potential_function(&rm), p0.data(), &cosmo, &frame, &lens, n, nx, 1, zs);
+    st();
+    put(p0);
+    map_grid_dpl_GPU(nullptr, p0.data(), p1.data(), &cosmo, &frame, &lens, n, nx, 1, zs);
+    st();
+    put(p0);
+    put(p1);
+    FILE* o = std::fopen(argv[2], "wb");
+    if (!o) return 5;
+    std::fwrite(out.data(), 8, out.size(), o);
+    std::fwrite(&worst, 4, 1, o);
+    std::fclose(o);
+    return 0;
+}

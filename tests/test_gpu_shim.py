@@ -1,0 +1,65 @@
+"""The reference-signature C++ entry points (include/lenstool_hip/lenstool_gpu.hpp), driven by a
+small C++ program that uses them the way a reference caller does (tests/cpp/shim_check.cpp)."""
+import os
+import struct
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import ltmodel as M
+from conftest import ROOT
+
+pytestmark = pytest.mark.gpu
+BIN = os.path.join(ROOT, "tests", "cpp", "shim_check")
+
+
+def test_reference_signatures(oracle):
+    import lthip
+    lthip.check(lthip.lib().lt_init(0))
+    if not os.path.exists(BIN):
+        pytest.fail("tests/cpp/shim_check is not built (make -C lenstool-hpc_amd/csrc)")
+    lens = M.config1_lens(seed=9)
+    cosmo = M.flat_lcdm()
+    zs = 2.0
+    lens.dlsds[:] = oracle.cosmo_dlsds(0.4, 1.7, cosmo)
+    lens._soa = None
+    n = 72
+    grid = M.config1_grid(n)
+    with tempfile.TemporaryDirectory() as d:
+        fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(struct.pack("<3i5d", lens.n, n, n, -100.0, 100.0, -100.0, 100.0, zs))
+            f.write(lens.type.astype("<i4").tobytes())
+            for a in (lens.x, lens.y, lens.b0, lens.angle, lens.epot, lens.rcore, lens.rcut, lens.z, lens.dlsds):
+                f.write(a.astype("<f8").tobytes())
+        print(f"[case] shim_check config1 {lens.n} halos {n}x{n}", flush=True)
+        subprocess.run([BIN, fin, fout], check=True, timeout=120)
+        raw = open(fout, "rb").read()
+    status = struct.unpack("<i", raw[-4:])[0]
+    v = np.frombuffer(raw[:-4], dtype="<f8").reshape(-1, n, n)
+    assert status == 0
+    assert v.shape[0] == 13
+    gx, gy = lthip.gradient_grid(lens, grid)
+    np.testing.assert_array_equal(v[0], gx)
+    np.testing.assert_array_equal(v[1], gy)
+    H = lthip.hessian_grid(lens, grid, 1, 2)
+    for k in range(4):
+        np.testing.assert_array_equal(v[2 + k], H[k])
+    mu = lthip.map_grid(lthip.MAP_AMPLI, 1, lens, grid, cosmo, 0.4, zs)
+    np.testing.assert_array_equal(v[6], mu)
+    # the foreign map function runs the epilogue on the stored planes: equal up to the rounding of
+    # a = ((a + c) + (a - c)) / 2, amplified by |mu| near the critical curves
+    assert (np.abs(v[7] - mu) <= 1e-12 * np.abs(mu) * np.maximum(1.0, np.abs(mu))).all()
+    np.testing.assert_array_equal(v[8], lthip.map_grid(lthip.MAP_SHEAR, 3, lens, grid, cosmo, 0.4, zs))
+    np.testing.assert_array_equal(v[9], lthip.map_grid(lthip.MAP_MASS, 2, lens, grid, cosmo, 0.0, zs))
+    np.testing.assert_array_equal(v[10], lthip.map_grid(lthip.MAP_POT, 1, lens, grid, cosmo, 0.4, zs))
+    dx, dy = lthip.map_grid(lthip.MAP_DPL, 1, lens, grid, cosmo, 0.4, zs)
+    np.testing.assert_array_equal(v[11], dx)
+    np.testing.assert_array_equal(v[12], dy)
+    # and the shim's gradient is the oracle's within the FP64 criterion
+    ex, ey = oracle.gradient_grid(lens, grid, extended=True)
+    _, _, ax, ay = oracle.gradient_grid(lens, grid, with_abs=True)
+    cg, _ = oracle.cond_planes(lens, grid)
+    assert (np.abs(v[0] - ex) <= 1e-12 * ax + cg).all()
