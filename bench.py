@@ -7,8 +7,11 @@ written in the same kernel, the lens model and the five output planes resident i
 halo x pixel evaluations per second over the timed steps; per-step latencies come from HIP events on
 the stream the kernel runs on.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline] [--precision fp64|fp32c]
 
+The headline value is the FP64 map (--precision fp64, default); the same steps are then timed in
+the other precision and reported under "fp32c" (or "fp64"), with the FP32-compensated map's maximum
+absolute error against the FP64 map (det A, kappa, gamma1, gamma2) measured after the timed region.
 Prints one JSON line. Runs on one GPU only (--gpus > 1 exits with a message).
 """
 import argparse
@@ -29,6 +32,7 @@ import ltmodel as M  # noqa: E402
 import lthip  # noqa: E402
 
 PEAK_FP64_TFLOPS = 78.6   # MI355X vector FP64
+PEAK_FP32_TFLOPS = 157.3  # MI355X vector FP32 (published figure, counts packed FP32)
 PEAK_HBM_GBS = 8000.0
 METRIC = "halo·pixel evals/s, Hessian maps (FP32-comp vs FP64 GPU); p50/p99 map latency"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hessian_fields_fp64_4096_503halos_traffic.md")
@@ -104,6 +108,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--precision", choices=("fp64", "fp32c"), default="fp64")
+    ap.add_argument("--no-secondary", action="store_true", help="do not time the other precision")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="also run one device copy kernel of this many bytes (hardware-counter calibration)")
     a = ap.parse_args()
@@ -122,13 +128,16 @@ def main():
     outs = [lthip.DeviceArray(grid.npix) for _ in range(5)]
     stream = lthip.stream_create()
 
-    per, total_ms, wall = run_maps(model, outs, grid, r, lthip.PREC_FP64, a.steps, a.warmup, stream)
+    prec = lthip.PREC_FP64 if a.precision == "fp64" else lthip.PREC_FP32C
+    other = lthip.PREC_FP32C if prec == lthip.PREC_FP64 else lthip.PREC_FP64
+    per, total_ms, wall = run_maps(model, outs, grid, r, prec, a.steps, a.warmup, stream)
     halo_px = lens.n * grid.npix
     value = a.steps * halo_px / (total_ms * 1e-3)
     avg_ms = float(np.mean(per))
     flops_per_map = lens.halo_flops("hessian") * grid.npix
     achieved = flops_per_map / (avg_ms * 1e-3) / 1e12
-    traffic = read_traffic()
+    peak = PEAK_FP64_TFLOPS if prec == lthip.PREC_FP64 else PEAK_FP32_TFLOPS
+    traffic = read_traffic() if prec == lthip.PREC_FP64 else None
 
     res = {
         "metric": METRIC,
@@ -141,12 +150,12 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if prec == lthip.PREC_FP64 else "f32+f64",
         "data": "synthetic (seeded lens model and grid; no external data)",
         "config": {
             "workload": "cluster Hessian + kappa/gamma1/gamma2/detA/mu maps, 4096x4096 px over 300 arcsec, "
                         "503 dPIE halos (2 cluster + 1 BCG + 500 luminosity-scaled galaxies), z_l=0.4, z_s=2.0",
-            "pixels": grid.npix, "halos": lens.n, "halo_px_per_step": halo_px, "precision": "fp64",
+            "pixels": grid.npix, "halos": lens.n, "halo_px_per_step": halo_px, "precision": a.precision,
             "parallelism": "single GPU",
         },
         "latency_ms": {"p50": percentile(per, 50), "p99": percentile(per, 99), "mean": avg_ms},
@@ -154,13 +163,26 @@ def main():
         "roofline": {
             "bound": "valu",
             "achieved": achieved,
-            "peak": PEAK_FP64_TFLOPS,
+            "peak": peak,
             "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP64_TFLOPS,
+            "frac": achieved / peak,
             "traffic": traffic,
             "flops_convention": "algorithmic: 262 flop per dPIE Hessian halo*px (div, sqrt = 8)",
         },
     }
+    if not a.no_secondary:
+        # the other precision on the same steps, then the FP32C error against FP64 (untimed)
+        first = [o.download() for o in outs[:4]]
+        per2, total2, _ = run_maps(model, outs, grid, r, other, a.steps, a.warmup, stream)
+        second = [o.download() for o in outs[:4]]
+        f64, f32 = (first, second) if prec == lthip.PREC_FP64 else (second, first)
+        errs = {n: float(np.abs(x - y).max()) for n, x, y in zip(("kappa", "gamma1", "gamma2", "detA"), f32, f64)}
+        name = "fp32c" if other == lthip.PREC_FP32C else "fp64"
+        v2 = a.steps * halo_px / (total2 * 1e-3)
+        res[name] = {"value": v2, "unit": "halo·px/s", "ms_per_step": total2 / a.steps,
+                     "latency_ms": {"p50": percentile(per2, 50), "p99": percentile(per2, 99)},
+                     "fp32c_over_fp64": (v2 / value) if name == "fp32c" else (value / v2),
+                     "fp32c_max_abs_err_vs_fp64": errs}
     if a.copy_bytes > 0:
         src = lthip.DeviceArray(a.copy_bytes, np.uint8)
         dst = lthip.DeviceArray(a.copy_bytes, np.uint8)

@@ -114,8 +114,22 @@ typedef struct lt_model_s* lt_model;
  * SHEAR, FIELDS), LT_MAP_MASS (Kmap runs) or LT_MAP_POT. */
 int lt_model_create(lt_model* out, const lt_potential_soa* lens, int nhalos, int use);
 /* Replace the parameters of a model in place (same halo count and types), e.g. per Bayes sample;
- * ordered on `stream` after the work already queued there. */
+ * ordered on `stream` after the work already queued there. Returns once the parameters are packed
+ * into the model's pinned staging buffer; the upload itself is asynchronous. */
 int lt_model_update(lt_model model, const lt_potential_soa* lens, int nhalos, void* stream);
+/* A batch of nsamples lens models with the same halo types (and, for LT_MAP_MASS, the same dlsds
+ * runs), e.g. Bayes samples of one model (the reference's per-sample loop in
+ * utils/maps/main.cpp:428-587). One lt_map_device call then computes every sample's map in one
+ * launch; sample s writes its planes at out + s * pitch * ny. */
+int lt_model_create_batch(lt_model* out, const lt_potential_soa* lenses, int nsamples, int nhalos, int use);
+int lt_model_update_batch(lt_model model, const lt_potential_soa* lenses, int nsamples, void* stream);
+/* The two halves of an update, for captured launch sequences: lt_model_stage packs new parameters
+ * into the pinned staging buffer (host only; it first waits for the previous upload to have read
+ * it), lt_model_upload queues the staging -> device copy on a stream and may be captured into a
+ * graph, so that each replay uploads whatever was staged last. A caller replaying such a graph
+ * waits for the replay to finish before staging again. */
+int lt_model_stage(lt_model model, const lt_potential_soa* lenses, int nsamples);
+int lt_model_upload(lt_model model, void* stream);
 int lt_model_destroy(lt_model model);
 /* device bytes held by a model */
 int lt_model_bytes(lt_model model, long long* bytes);
@@ -134,6 +148,16 @@ typedef struct lt_grid_desc {
 int lt_map_device(lt_model model, int kind, int mode, double* out0, double* out1, double* out2, double* out3,
                   double* out4, double r, double mult, const lt_grid_desc* grid, int precision, int px,
                   void* stream);
+
+/* ---- captured launch sequences (HIP graphs) ------------------------------------------------------
+ * Work issued on a caller stream between lt_graph_begin and lt_graph_end (lt_model_upload,
+ * lt_map_device, ...) is captured, not run; lt_graph_launch replays it on any stream with one
+ * submission. The captured calls issue no memset nodes. */
+typedef struct lt_graph_s* lt_graph;
+int lt_graph_begin(void* stream);
+int lt_graph_end(void* stream, lt_graph* out);
+int lt_graph_launch(lt_graph graph, void* stream);
+int lt_graph_destroy(lt_graph graph);
 
 /* ---- point evaluations, delensing and chi^2 ------------------------------------------------------ */
 

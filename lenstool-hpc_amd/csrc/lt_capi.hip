@@ -1,6 +1,7 @@
 // lt_capi.hip -- implementation of the C-ABI (include/lenstool_hip/lt_capi.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -117,8 +118,8 @@ int launch_map(const DevModel& m, int kind, int mode, double* const outs[5], dou
     o.mult = mult;
     if (px <= 0) px = default_px(kind, precision);
     if (px != 1 && px != 2 && px != 4) return fail(LT_ERR_INVALID, "px must be 0, 1, 2 or 4");
-    if (precision != LT_PREC_FP64)
-        return fail(LT_ERR_INVALID, "precision: only LT_PREC_FP64 is available in this build");
+    if (precision != LT_PREC_FP64 && precision != LT_PREC_FP32C)
+        return fail(LT_ERR_INVALID, "precision must be LT_PREC_FP64 or LT_PREC_FP32C");
     hipError_t e;
     switch (kind) {
     case LT_MAP_GRADIENT:
@@ -129,7 +130,9 @@ int launch_map(const DevModel& m, int kind, int mode, double* const outs[5], dou
         e = launch_potential64(m.recs, m.runs, m.nruns, g, o, s);
         break;
     default:
-        e = launch_hessian64(m.recs, m.runs, m.nruns, g, o, px, s);
+        // FP32C covers the Hessian-derived maps; gradient and potential maps are FP64 either way
+        if (precision == LT_PREC_FP32C) e = launch_hessian32(m.recs, m.recs32, m.runs, m.nruns, g, o, px, s);
+        else e = launch_hessian64(m.recs, m.runs, m.nruns, g, o, px, s);
         break;
     }
     if (e != hipSuccess) return hip_fail(e, "kernel launch");
@@ -503,11 +506,16 @@ struct lt_model_s {
     int device = 0;
     Use use = Use::Hessian;
     int nhalos = 0;
+    int nsamples = 1;
     std::vector<Run> runs_host;
     std::vector<int> types;
     DevModel dm;
     size_t bytes = 0;
-    void* base = nullptr;
+    size_t rec_bytes = 0, rec32_bytes = 0;  // all samples
+    void* base = nullptr;                   // device: records [S*H], fp32 records [S*H], runs
+    void* staging = nullptr;                // pinned host image of the two record arrays
+    hipEvent_t staged = nullptr;            // completes when the last upload has read `staging`
+    bool staged_pending = false;
 };
 
 static Use use_from_kind(int use)
@@ -521,12 +529,79 @@ static Use use_from_kind(int use)
     }
 }
 
+static bool same_runs(const std::vector<Run>& a, const std::vector<Run>& b)
+{
+    if (a.size() != b.size()) return false;
+    for (size_t i = 0; i < a.size(); ++i)
+        if (a[i].type != b[i].type || a[i].start != b[i].start || a[i].count != b[i].count || a[i].weight != b[i].weight)
+            return false;
+    return true;
+}
+
+// Pack every sample into the pinned staging image (after the previous upload has read it).
+static int stage_samples(lt_model_s* m, const lt_potential_soa* lenses, int nsamples)
+{
+    if (!lenses) return fail(LT_ERR_INVALID, "lens models are NULL");
+    if (nsamples != m->nsamples) return fail(LT_ERR_INVALID, "sample count differs from the model's");
+    if (m->staged_pending) {
+        HIP_TRY(hipEventSynchronize(m->staged));
+        m->staged_pending = false;
+    }
+    const int H = m->nhalos;
+    std::vector<HaloRec> recs;
+    std::vector<HaloRec32> recs32;
+    std::vector<Run> runs;
+    std::string err;
+    char* st = (char*)m->staging;
+    for (int s = 0; s < nsamples; ++s) {
+        const lt_potential_soa& L = lenses[s];
+        for (int i = 0; i < H; ++i)
+            if (!L.type || L.type[i] != m->types[i])
+                return fail(LT_ERR_INVALID, "sample " + std::to_string(s) + ": halo types differ from the model's");
+        int rc = pack_model(&L, H, m->use, recs, recs32, runs, err);
+        if (rc) return fail(rc, "sample " + std::to_string(s) + ": " + err);
+        if (!same_runs(runs, m->runs_host))
+            return fail(LT_ERR_INVALID, "sample " + std::to_string(s) + ": run structure (types / dlsds) differs");
+        if (H) {
+            std::memcpy(st + (size_t)s * H * sizeof(HaloRec), recs.data(), (size_t)H * sizeof(HaloRec));
+            std::memcpy(st + m->rec_bytes + (size_t)s * H * sizeof(HaloRec32), recs32.data(),
+                        (size_t)H * sizeof(HaloRec32));
+        }
+    }
+    return LT_OK;
+}
+
+// Queue the copy staging -> device on `s`.
+static int upload_staged(lt_model_s* m, hipStream_t s)
+{
+    const size_t n = m->rec_bytes + m->rec32_bytes;
+    if (!n) return LT_OK;
+    HIP_TRY(hipMemcpyAsync(m->base, m->staging, n, hipMemcpyHostToDevice, s));
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIP_TRY(hipStreamIsCapturing(s, &cs));
+    if (cs == hipStreamCaptureStatusNone) {
+        HIP_TRY(hipEventRecord(m->staged, s));
+        m->staged_pending = true;
+    }
+    return LT_OK;
+}
+
+static void free_model(lt_model_s* m)
+{
+    if (m->staged) (void)hipEventDestroy(m->staged);
+    if (m->staging) (void)hipHostFree(m->staging);
+    if (m->base) (void)hipFree(m->base);
+    delete m;
+}
+
 extern "C" {
 
-int lt_model_create(lt_model* out, const lt_potential_soa* lens, int nhalos, int use)
+int lt_model_create_batch(lt_model* out, const lt_potential_soa* lenses, int nsamples, int nhalos, int use)
 {
     if (!out) return fail(LT_ERR_INVALID, "out is NULL");
     *out = nullptr;
+    if (!lenses || nsamples < 1) return fail(LT_ERR_INVALID, "need at least one lens model");
+    if (nhalos < 0) return fail(LT_ERR_INVALID, "negative halo count");
     int st = check_kind_mode(use, use == LT_MAP_AMPLI || use == LT_MAP_SHEAR || use == LT_MAP_MASS ||
                                       use == LT_MAP_POT ? 1 : 0);
     if (st) return st;
@@ -535,76 +610,90 @@ int lt_model_create(lt_model* out, const lt_potential_soa* lens, int nhalos, int
     std::vector<Run> runs;
     std::string err;
     Use u = use_from_kind(use);
-    if ((st = pack_model(lens, nhalos, u, recs, recs32, runs, err))) return fail(st, err);
+    if ((st = pack_model(&lenses[0], nhalos, u, recs, recs32, runs, err))) return fail(st, "sample 0: " + err);
     int dev;
     if ((st = current_device(&dev))) return st;
     lt_model_s* m = new lt_model_s();
     m->device = dev;
     m->use = u;
     m->nhalos = nhalos;
+    m->nsamples = nsamples;
     m->runs_host = runs;
-    for (int i = 0; i < nhalos; ++i) m->types.push_back(lens->type[i]);
-    const size_t rb = recs.size() * sizeof(HaloRec), r32 = recs32.size() * sizeof(HaloRec32);
-    const size_t off_runs = ((rb + r32 + 63) / 64) * 64;
+    for (int i = 0; i < nhalos; ++i) m->types.push_back(lenses[0].type[i]);
+    m->rec_bytes = (size_t)nsamples * nhalos * sizeof(HaloRec);
+    m->rec32_bytes = (size_t)nsamples * nhalos * sizeof(HaloRec32);
+    const size_t off_runs = ((m->rec_bytes + m->rec32_bytes + 63) / 64) * 64;
     m->bytes = off_runs + runs.size() * sizeof(Run) + 64;
     hipError_t e = hipMalloc(&m->base, m->bytes);
+    if (e == hipSuccess) e = hipHostMalloc(&m->staging, std::max<size_t>(m->rec_bytes + m->rec32_bytes, 64));
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&m->staged, hipEventDisableTiming);
     if (e != hipSuccess) {
         (void)hipGetLastError();
-        delete m;
+        free_model(m);
         return fail(LT_ERR_NOMEM, "model allocation failed");
     }
     char* b = (char*)m->base;
     m->dm.recs = (HaloRec*)b;
-    m->dm.recs32 = (HaloRec32*)(b + rb);
+    m->dm.recs32 = (HaloRec32*)(b + m->rec_bytes);
     m->dm.runs = (Run*)(b + off_runs);
     m->dm.nruns = (int)runs.size();
-    e = hipSuccess;
-    if (rb && e == hipSuccess) e = hipMemcpy(m->dm.recs, recs.data(), rb, hipMemcpyHostToDevice);
-    if (r32 && e == hipSuccess) e = hipMemcpy(m->dm.recs32, recs32.data(), r32, hipMemcpyHostToDevice);
+    if ((st = stage_samples(m, lenses, nsamples))) {
+        free_model(m);
+        return st;
+    }
+    e = hipMemcpy(m->base, m->staging, m->rec_bytes + m->rec32_bytes, hipMemcpyHostToDevice);
     if (!runs.empty() && e == hipSuccess)
         e = hipMemcpy(m->dm.runs, runs.data(), runs.size() * sizeof(Run), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
-        (void)hipFree(m->base);
-        delete m;
+        free_model(m);
         return hip_fail(e, "model upload");
     }
     *out = m;
     return LT_OK;
 }
 
+int lt_model_create(lt_model* out, const lt_potential_soa* lens, int nhalos, int use)
+{
+    return lt_model_create_batch(out, lens, 1, nhalos, use);
+}
+
+int lt_model_update_batch(lt_model m, const lt_potential_soa* lenses, int nsamples, void* stream)
+{
+    if (!m) return fail(LT_ERR_NOT_FOUND, "model is NULL");
+    HIP_TRY(hipSetDevice(m->device));
+    int st = stage_samples(m, lenses, nsamples);
+    if (st) return st;
+    return upload_staged(m, (hipStream_t)stream);
+}
+
 int lt_model_update(lt_model m, const lt_potential_soa* lens, int nhalos, void* stream)
 {
     if (!m) return fail(LT_ERR_NOT_FOUND, "model is NULL");
     if (nhalos != m->nhalos) return fail(LT_ERR_INVALID, "halo count differs from the model's");
-    for (int i = 0; i < nhalos; ++i)
-        if (!lens || !lens->type || lens->type[i] != m->types[i])
-            return fail(LT_ERR_INVALID, "halo types differ from the model's");
-    std::vector<HaloRec> recs;
-    std::vector<HaloRec32> recs32;
-    std::vector<Run> runs;
-    std::string err;
-    int st = pack_model(lens, nhalos, m->use, recs, recs32, runs, err);
-    if (st) return fail(st, err);
-    if (runs.size() != m->runs_host.size()) return fail(LT_ERR_INVALID, "run structure differs from the model's");
+    if (m->nsamples != 1) return fail(LT_ERR_INVALID, "batched model: use lt_model_update_batch");
+    return lt_model_update_batch(m, lens, 1, stream);
+}
+
+int lt_model_stage(lt_model m, const lt_potential_soa* lenses, int nsamples)
+{
+    if (!m) return fail(LT_ERR_NOT_FOUND, "model is NULL");
     HIP_TRY(hipSetDevice(m->device));
-    hipStream_t s = (hipStream_t)stream;
-    const size_t rb = recs.size() * sizeof(HaloRec), r32 = recs32.size() * sizeof(HaloRec32);
-    if (rb) HIP_TRY(hipMemcpyAsync(m->dm.recs, recs.data(), rb, hipMemcpyHostToDevice, s));
-    if (r32) HIP_TRY(hipMemcpyAsync(m->dm.recs32, recs32.data(), r32, hipMemcpyHostToDevice, s));
-    if (!runs.empty()) HIP_TRY(hipMemcpyAsync(m->dm.runs, runs.data(), runs.size() * sizeof(Run), hipMemcpyHostToDevice, s));
-    // the host vectors die here: make the copies complete before returning
-    HIP_TRY(hipStreamSynchronize(s));
-    m->runs_host = runs;
-    return LT_OK;
+    return stage_samples(m, lenses, nsamples);
+}
+
+int lt_model_upload(lt_model m, void* stream)
+{
+    if (!m) return fail(LT_ERR_NOT_FOUND, "model is NULL");
+    HIP_TRY(hipSetDevice(m->device));
+    return upload_staged(m, (hipStream_t)stream);
 }
 
 int lt_model_destroy(lt_model m)
 {
     if (!m) return LT_OK;
     (void)hipSetDevice(m->device);
-    hipError_t e = hipFree(m->base);
-    delete m;
-    if (e != hipSuccess) return hip_fail(e, "model free");
+    if (m->staged_pending) (void)hipEventSynchronize(m->staged);
+    free_model(m);
     return LT_OK;
 }
 
@@ -642,7 +731,9 @@ int lt_map_device(lt_model m, int kind, int mode, double* out0, double* out1, do
     g.istart = grid->istart;
     g.jstart = grid->jstart;
     g.pitch = grid->pitch;
-    g.nsamples = 1;
+    g.nsamples = m->nsamples;
+    g.halo_stride = m->nhalos;
+    g.plane_stride = grid->pitch * (long long)grid->ny;
     HIP_TRY(hipSetDevice(m->device));
     return launch_map(m->dm, kind, mode, outs, r, mult, g, precision, px, (hipStream_t)stream);
 }
@@ -685,6 +776,61 @@ int lt_map_epilogue_device(int kind, int mode, double* out0, double* out1, doubl
     hipError_t e = launch_epilogue64(a, b, c, (int64_t)n, o, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "epilogue launch");
     if (!stream) HIP_TRY(hipStreamSynchronize(nullptr));
+    return LT_OK;
+}
+
+// ---- captured launch sequences -------------------------------------------------------------------
+struct lt_graph_s {
+    int device = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+int lt_graph_begin(void* stream)
+{
+    if (!stream) return fail(LT_ERR_INVALID, "capture needs a caller stream (not the NULL stream)");
+    HIP_TRY(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal));
+    return LT_OK;
+}
+
+int lt_graph_end(void* stream, lt_graph* out)
+{
+    if (!out) return fail(LT_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (!stream) return fail(LT_ERR_INVALID, "capture needs a caller stream (not the NULL stream)");
+    int dev, st;
+    if ((st = current_device(&dev))) return st;
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamEndCapture((hipStream_t)stream, &g));
+    hipGraphExec_t x = nullptr;
+    hipError_t e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return hip_fail(e, "graph instantiate");
+    }
+    lt_graph_s* r = new lt_graph_s();
+    r->device = dev;
+    r->graph = g;
+    r->exec = x;
+    *out = r;
+    return LT_OK;
+}
+
+int lt_graph_launch(lt_graph g, void* stream)
+{
+    if (!g) return fail(LT_ERR_NOT_FOUND, "graph is NULL");
+    HIP_TRY(hipSetDevice(g->device));
+    HIP_TRY(hipGraphLaunch(g->exec, (hipStream_t)stream));
+    return LT_OK;
+}
+
+int lt_graph_destroy(lt_graph g)
+{
+    if (!g) return LT_OK;
+    (void)hipSetDevice(g->device);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
     return LT_OK;
 }
 

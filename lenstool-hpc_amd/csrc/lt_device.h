@@ -46,19 +46,23 @@ struct alignas(32) Run {
 };
 static_assert(sizeof(Run) == 32, "run is 32 bytes");
 
-// Single-precision image of a halo record for the FP32-compensated kernels: the same constants
-// rounded to float, plus the centre split into hi + lo floats so that pixel - centre is formed
-// with FP64 accuracy in FP32 arithmetic.
+// Single-precision record of a halo for the FP32-compensated Hessian. The PIEMD terms use a form
+// of the I_{0.5} Hessian without the removable 0/0 of the closed form (see lt_kernels32.hip), whose
+// constants are these. The centre is split into hi + lo floats so that pixel - centre keeps FP64
+// accuracy in FP32 arithmetic. near2: squared halo-frame radius inside which the FP32 rounding of
+// this halo's term would exceed the error budget; there the FP64 term of the HaloRec is used.
 struct alignas(64) HaloRec32 {
     float xh_hi, xh_lo, yh_hi, yh_lo;   // 0..3
-    float cs, sn, c2, s2;               // 4..7
-    float kx, ky, e2, cx1;              // 8..11
-    float icx1, cx1sq, ra2, rb2;        // 12..15
-    float e2ra, e2rb, ampg, amph;       // 16..19
-    float hk1, hk2, hk3, hk4;           // 20..23
-    float hk5, hk6, sa, sb;             // 24..27
-    float h14a, h14b, h14c;             // 28..30
-    float near2;                        // 31: squared radius inside which this halo is taken in FP64
+    float cs, sn;                       // 4, 5
+    float a2, b2;                       // 6, 7    a^2 = 1/(1+e)^2, b^2 = 1/(1-e)^2
+    float e2, cx1, e2ab, e2a2;          // 8..11   2 sqrt(e), (1-e)/(1+e), 2 sqrt(e) a b, 2 sqrt(e) a^2
+    float cxa2;                         // 12      cx1 a^2
+    float ra2, e2ra, rcx1a;             // 13..15  rcore^2, 2 rcore sqrt(e), rcore cx1
+    float rb2, e2rb, rcx1b;             // 16..18  the same for rcut
+    float hT, hc2, hs2m2, hs2h;         // 19..22  lab-frame amplitudes: trace, cos 2t, -2 sin 2t, sin 2t / 2
+    float near2;                        // 23
+    float sa, sb;                       // 24, 25  SIS: 1 - e, 1 + e
+    float pad[6];                       // 26..31
 };
 static_assert(sizeof(HaloRec32) == 128, "fp32 halo record is 128 bytes");
 

@@ -1,0 +1,254 @@
+// lt_kernels32.hip -- FP32-compensated Hessian (+ fused map epilogue) for gfx950.
+//
+// Same launch shape and run walk as the FP64 kernel (lt_kernels64.hip). Per (halo, pixel):
+//   * the halo-frame offset is formed from hi + lo float splits of the pixel and the centre, so it
+//     carries FP64 accuracy into FP32;
+//   * inside the halo's near radius (HaloRec32::near2: cores, where the term is large) the FP64 term
+//     is used, outside it the FP32 term below;
+//   * FP32 terms are summed in FP32 over chunks of CHUNK halos and each chunk sum is added to FP64
+//     run sums (two-level compensated summation: the FP32 sum error is bounded by CHUNK u sum|t|).
+//
+// FP32 form of the PIEMD I_{0.5} Hessian. With s = sqrt(e), a = 1/(1+e), b = 1/(1-e), cx1 = a/b,
+// w = sqrt(r^2 + a^2 x^2 + b^2 y^2), D = x + i (2 r s - y) and N = cx1 x + i (2 s w - y / cx1), the
+// reference's closed form (src/gradient2.cpp:54-82) is the Hessian of Im/Re ln(N / D), whose two
+// pieces N'/N and D'/D are both singular at D = 0, i.e. at halo-frame (0, 2 r s); the closed form
+// cancels them there and in FP32 loses ~1e-5 relative over the whole upper half plane. Writing
+// v = r cx1 + 2 s a b y, one has w^2 - v^2 = a^2 |D|^2 and N = cx1 D + 2 i s (w - v), hence
+//   N / D = Q / S,  S = w + v,  Q = cx1 S + 2 s a^2 (n2 + i x),  n2 = 2 r s - y,
+// and with U = b^2 y / w + 2 s a b the three second derivatives are, up to the factor 2 s a^2 ci:
+//   phi_xx ~ (Re Q - cx1 a^2 x^2 / w) / |Q|^2
+//   phi_xy ~ x (2 s a^2 - cx1 U) / |Q|^2
+//   phi_yy ~ (Re Q + U (cx1 n2 + 2 s (w - v))) / |Q|^2
+// Q never vanishes and S > 0, so the form has no cancellation near D = 0; it needs one rsq and one
+// rcp per core radius. (Derivation: differentiate ln Q - ln S with dw/dx = a^2 x / w,
+// dw/dy = b^2 y / w, dv/dy = 2 s a b; |D|^2 / S = (w - v) / a^2.)
+#include <hip/hip_runtime.h>
+
+#include "lenstool_hip/lt_status.h"
+#include "lt_device.h"
+#include "lt_kernels.h"
+#include "lt_math.h"
+#include "lt_terms64.h"
+
+namespace lt {
+
+constexpr int CHUNK = 8;
+
+__device__ __forceinline__ float rsq32(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float rcp32(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// Halo-frame coordinates of a pixel in FP32 (pixel and centre both as hi + lo floats).
+struct Frame32 {
+    float x, y;
+};
+
+__device__ __forceinline__ Frame32 frame32(const HaloRec32& h, float pxh, float pxl, float pyh, float pyl)
+{
+    const float tx = (pxh - h.xh_hi) + (pxl - h.xh_lo);
+    const float ty = (pyh - h.yh_hi) + (pyl - h.yh_lo);
+    Frame32 f;
+    f.x = __builtin_fmaf(tx, h.cs, ty * h.sn);
+    f.y = __builtin_fmaf(ty, h.cs, -(tx * h.sn));
+    return f;
+}
+
+struct Q32 {
+    float xx, xy, yy;
+};
+
+// pixel-only quantities shared by the core and cut terms
+struct QShared {
+    float wq0, e2aby, Qi2, cxa2x2, b2y, e2a2x, cx1x;
+};
+
+__device__ __forceinline__ QShared q_shared(const HaloRec32& h, float x, float y)
+{
+    QShared s;
+    const float x2 = x * x;
+    s.wq0 = __builtin_fmaf(h.a2, x2, h.b2 * (y * y));
+    s.e2aby = h.e2ab * y;
+    s.e2a2x = h.e2a2 * x;
+    s.Qi2 = s.e2a2x * s.e2a2x;
+    s.cxa2x2 = h.cxa2 * x2;
+    s.b2y = h.b2 * y;
+    s.cx1x = h.cx1 * x;
+    return s;
+}
+
+__device__ __forceinline__ Q32 q_term(const HaloRec32& h, const QShared& s, float y, float r2, float e2r,
+                                      float rcx1)
+{
+    const float w2 = s.wq0 + r2;
+    const float iw = rsq32(w2);
+    const float w = w2 * iw;
+    const float v = s.e2aby + rcx1;
+    const float n2 = e2r - y;
+    const float Qr = __builtin_fmaf(h.e2a2, n2, h.cx1 * (w + v));
+    const float iq = rcp32(__builtin_fmaf(Qr, Qr, s.Qi2));
+    const float U = __builtin_fmaf(s.b2y, iw, h.e2ab);
+    Q32 o;
+    o.xx = __builtin_fmaf(-s.cxa2x2, iw, Qr) * iq;
+    o.xy = __builtin_fmaf(-s.cx1x, U, s.e2a2x) * iq;
+    o.yy = __builtin_fmaf(U, __builtin_fmaf(h.e2, w - v, h.cx1 * n2), Qr) * iq;
+    return o;
+}
+
+// add one halo-frame Hessian (xx, xy, yy), already differenced over the core radii, in the lab frame
+__device__ __forceinline__ void acc_lab(const HaloRec32& h, float xx, float xy, float yy, float& fT, float& fD,
+                                        float& fB)
+{
+    const float T = xx + yy, D = xx - yy;
+    fT = __builtin_fmaf(h.hT, T, fT);
+    fD = __builtin_fmaf(h.hc2, D, __builtin_fmaf(h.hs2m2, xy, fD));
+    fB = __builtin_fmaf(h.hs2h, D, __builtin_fmaf(h.hc2, xy, fB));
+}
+
+template <bool CUT>
+__device__ __forceinline__ void hess_piemd32(const HaloRec32& h, const Frame32& f, float& fT, float& fD, float& fB)
+{
+    const QShared s = q_shared(h, f.x, f.y);
+    Q32 a = q_term(h, s, f.y, h.ra2, h.e2ra, h.rcx1a);
+    if (CUT) {
+        const Q32 b = q_term(h, s, f.y, h.rb2, h.e2rb, h.rcx1b);
+        a.xx -= b.xx;
+        a.xy -= b.xy;
+        a.yy -= b.yy;
+    }
+    acc_lab(h, a.xx, a.xy, a.yy, fT, fD, fB);
+}
+
+// SIS / SIE: halo-frame Hessian amph / R^3 (y^2, -xy, x^2), R^2 = (1-e) x^2 + (1+e) y^2
+__device__ __forceinline__ void hess_sis32(const HaloRec32& h, const Frame32& f, float& fT, float& fD, float& fB)
+{
+    const float x2 = f.x * f.x, y2 = f.y * f.y;
+    const float iR = rsq32(__builtin_fmaf(h.sa, x2, h.sb * y2));
+    const float g = iR * iR * iR;
+    acc_lab(h, g * y2, -(g * f.x * f.y), g * x2, fT, fD, fB);
+}
+
+// One run of halos of one type: FP32 terms summed per chunk of CHUNK halos, FP64 terms (near
+// lanes) straight into the FP64 run sums.
+template <int TYPE, int PX>
+__device__ __forceinline__ void run32(const HaloRec* __restrict__ h64, const HaloRec32* __restrict__ h32, int start,
+                                      int end, const double* px, double py, const float* pxh, const float* pxl,
+                                      float pyh, float pyl, double* sT, double* sD, double* sB)
+{
+    for (int c0 = start; c0 < end; c0 += CHUNK) {
+        const int c1 = min(c0 + CHUNK, end);
+        float fT[PX], fD[PX], fB[PX];
+#pragma unroll
+        for (int k = 0; k < PX; ++k) fT[k] = fD[k] = fB[k] = 0.f;
+        for (int i = c0; i < c1; ++i) {
+            const HaloRec32& h = h32[i];
+#pragma unroll
+            for (int k = 0; k < PX; ++k) {
+                const Frame32 f = frame32(h, pxh[k], pxl[k], pyh, pyl);
+                if (__builtin_fmaf(f.x, f.x, f.y * f.y) < h.near2) {
+                    const HaloRec& hd = h64[i];
+                    if constexpr (TYPE == TYPE_DPIE) hess_piemd<true>(hd, px[k], py, sT[k], sD[k], sB[k]);
+                    else if constexpr (TYPE == TYPE_PIEMD) hess_piemd<false>(hd, px[k], py, sT[k], sD[k], sB[k]);
+                    else hess_sis(hd, px[k], py, sT[k], sD[k], sB[k]);
+                } else {
+                    if constexpr (TYPE == TYPE_DPIE) hess_piemd32<true>(h, f, fT[k], fD[k], fB[k]);
+                    else if constexpr (TYPE == TYPE_PIEMD) hess_piemd32<false>(h, f, fT[k], fD[k], fB[k]);
+                    else hess_sis32(h, f, fT[k], fD[k], fB[k]);
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < PX; ++k) {
+            sT[k] += (double)fT[k];
+            sD[k] += (double)fD[k];
+            sB[k] += (double)fB[k];
+        }
+    }
+}
+
+template <int PX, int KIND>
+__global__ __launch_bounds__(WAVE* ROWS) void k_hessian32(const HaloRec* __restrict__ h64,
+                                                          const HaloRec32* __restrict__ h32,
+                                                          const Run* __restrict__ runs, int nruns,
+                                                          GridDesc g, OutDesc o)
+{
+    const int row = blockIdx.y * ROWS + threadIdx.y;
+    if (row >= g.ny) return;
+    h64 = select_sample(g, h64, o);
+    h32 += (int64_t)blockIdx.z * g.halo_stride;
+    const int col0 = blockIdx.x * (WAVE * PX) + threadIdx.x;
+    const double py = pixel_coord(g.ymin, row + g.jstart, g.dy);
+    const float pyh = (float)py, pyl = (float)(py - (double)pyh);
+    double px[PX];
+    float pxh[PX], pxl[PX];
+#pragma unroll
+    for (int k = 0; k < PX; ++k) {
+        px[k] = pixel_coord(g.xmin, col0 + k * WAVE + g.istart, g.dx);
+        pxh[k] = (float)px[k];
+        pxl[k] = (float)(px[k] - (double)pxh[k]);
+    }
+    double tT[PX], tD[PX], tB[PX];
+#pragma unroll
+    for (int k = 0; k < PX; ++k) tT[k] = tD[k] = tB[k] = 0.;
+    for (int r = 0; r < nruns; ++r) {
+        const Run run = runs[r];
+        double sT[PX], sD[PX], sB[PX];
+#pragma unroll
+        for (int k = 0; k < PX; ++k) sT[k] = sD[k] = sB[k] = 0.;
+        const int end = run.start + run.count;
+        if (run.type == TYPE_SHEAR) {
+            for (int i = run.start; i < end; ++i) {
+                const HaloRec& h = h64[i];
+#pragma unroll
+                for (int k = 0; k < PX; ++k) {
+                    sT[k] += h.h14a + h.h14c;
+                    sD[k] += h.h14a - h.h14c;
+                    sB[k] += h.h14b;
+                }
+            }
+        } else if (run.type == TYPE_DPIE) {
+            run32<TYPE_DPIE, PX>(h64, h32, run.start, end, px, py, pxh, pxl, pyh, pyl, sT, sD, sB);
+        } else if (run.type == TYPE_PIEMD) {
+            run32<TYPE_PIEMD, PX>(h64, h32, run.start, end, px, py, pxh, pxl, pyh, pyl, sT, sD, sB);
+        } else {
+            run32<TYPE_SIS, PX>(h64, h32, run.start, end, px, py, pxh, pxl, pyh, pyl, sT, sD, sB);
+        }
+        const double w = run.weight;
+#pragma unroll
+        for (int k = 0; k < PX; ++k) {
+            tT[k] = __builtin_fma(w, sT[k], tT[k]);
+            tD[k] = __builtin_fma(w, sD[k], tD[k]);
+            tB[k] = __builtin_fma(w, sB[k], tB[k]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < PX; ++k) {
+        const int col = col0 + k * WAVE;
+        if (col < g.nx) hess_epilogue<KIND>(o, (int64_t)row * g.pitch + col, tT[k], tD[k], tB[k]);
+    }
+}
+
+template <int PX>
+static hipError_t launch_hessian32_px(const HaloRec* h64, const HaloRec32* h32, const Run* runs, int nruns,
+                                      const GridDesc& g, const OutDesc& o, hipStream_t s)
+{
+    dim3 blk(WAVE, ROWS, 1), grd = grid_for(g, PX);
+    switch (o.kind) {
+    case LT_MAP_FIELDS: hipLaunchKernelGGL((k_hessian32<PX, LT_MAP_FIELDS>), grd, blk, 0, s, h64, h32, runs, nruns, g, o); break;
+    case LT_MAP_AMPLI: hipLaunchKernelGGL((k_hessian32<PX, LT_MAP_AMPLI>), grd, blk, 0, s, h64, h32, runs, nruns, g, o); break;
+    case LT_MAP_SHEAR: hipLaunchKernelGGL((k_hessian32<PX, LT_MAP_SHEAR>), grd, blk, 0, s, h64, h32, runs, nruns, g, o); break;
+    case LT_MAP_MASS: hipLaunchKernelGGL((k_hessian32<PX, LT_MAP_MASS>), grd, blk, 0, s, h64, h32, runs, nruns, g, o); break;
+    default: hipLaunchKernelGGL((k_hessian32<PX, LT_MAP_HESSIAN>), grd, blk, 0, s, h64, h32, runs, nruns, g, o); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_hessian32(const HaloRec* h64, const HaloRec32* h32, const Run* runs, int nruns, const GridDesc& g,
+                            const OutDesc& o, int px, hipStream_t s)
+{
+    switch (px) {
+    case 1: return launch_hessian32_px<1>(h64, h32, runs, nruns, g, o, s);
+    case 4: return launch_hessian32_px<4>(h64, h32, runs, nruns, g, o, s);
+    default: return launch_hessian32_px<2>(h64, h32, runs, nruns, g, o, s);
+    }
+}
+
+}  // namespace lt

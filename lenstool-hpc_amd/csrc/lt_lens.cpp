@@ -11,6 +11,13 @@ namespace lt {
 
 static bool finite(double v) { return std::isfinite(v); }
 
+// Radius (in units of the halo's b0, times rcut/(rcut - rcore) for the dPIE) inside which the
+// FP32-compensated Hessian takes a halo's term in FP64. Measured FP32 rounding of one term is about
+// 1e-6 b0 t05 / r at halo-frame radius r, and 1.2e-7 b0 / r for the SIS: these radii keep every FP32
+// term under 5e-8 (PIEMD) and 1.2e-8 (SIS).
+constexpr double kFp32NearScalePiemd = 20.0;
+constexpr double kFp32NearScaleSis = 10.0;
+
 static int bad(std::string& err, int i, const char* what)
 {
     char buf[160];
@@ -123,32 +130,44 @@ int pack_model(const lt_potential_soa* L, int n, Use use, std::vector<HaloRec>& 
         f.yh_lo = (float)(y - (double)f.yh_hi);
         f.cs = (float)cs;
         f.sn = (float)sn;
-        f.c2 = (float)h.hc2;
-        f.s2 = (float)h.hs2;
-        f.kx = (float)h.kx;
-        f.ky = (float)h.ky;
-        f.e2 = (float)h.e2;
-        f.cx1 = (float)h.cx1;
-        f.icx1 = (float)h.icx1;
-        f.cx1sq = (float)h.cx1sq;
-        f.ra2 = (float)h.ra2;
-        f.rb2 = (float)h.rb2;
-        f.e2ra = (float)h.e2ra;
-        f.e2rb = (float)h.e2rb;
-        f.ampg = (float)h.ampg;
-        f.amph = (float)h.amph;
-        f.hk1 = (float)h.hk1;
-        f.hk2 = (float)h.hk2;
-        f.hk3 = (float)h.hk3;
-        f.hk4 = (float)h.hk4;
-        f.hk5 = (float)h.hk5;
-        f.hk6 = (float)h.hk6;
-        f.sa = (float)h.sa;
-        f.sb = (float)h.sb;
-        f.h14a = (float)h.h14a;
-        f.h14b = (float)h.h14b;
-        f.h14c = (float)h.h14c;
-        f.near2 = 0.f;
+        if (piemd) {
+            const double a = 1.0 / (1.0 + eps), b = 1.0 / (1.0 - eps), e2 = h.e2;
+            // The FP32 terms are the I_{0.5} Hessian times 1 / (2 sqrt(e) a^2): fold that factor back in.
+            const double amp = h.amph * e2 * a * a;
+            f.a2 = (float)(a * a);
+            f.b2 = (float)(b * b);
+            f.e2 = (float)e2;
+            f.cx1 = (float)h.cx1;
+            f.e2ab = (float)(e2 * a * b);
+            f.e2a2 = (float)(e2 * a * a);
+            f.cxa2 = (float)(h.cx1 * a * a);
+            f.ra2 = (float)h.ra2;
+            f.e2ra = (float)h.e2ra;
+            f.rcx1a = (float)(h.rc * h.cx1);
+            f.rb2 = (float)h.rb2;
+            f.e2rb = (float)h.e2rb;
+            f.rcx1b = (float)(h.rcut * h.cx1);
+            f.hT = (float)amp;
+            f.hc2 = (float)(amp * cos2);
+            f.hs2m2 = (float)(-2.0 * amp * sin2);
+            f.hs2h = (float)(0.5 * amp * sin2);
+            // FP32 rounding of one term is about 1e-6 b0 t05 / r (r = halo-frame radius) away from the
+            // core; inside R = 20 b0 t05 it would exceed 5e-8, so the FP64 term is used there.
+            const double t05 = (t == TYPE_DPIE) ? h.rcut / (h.rcut - h.rc) : 1.0;
+            const double R = kFp32NearScalePiemd * std::fabs(b0 * t05);
+            f.near2 = (float)std::fmin(R * R, 3.0e38);
+        } else if (t == TYPE_SIS) {
+            f.sa = (float)h.sa;
+            f.sb = (float)h.sb;
+            f.hT = (float)h.amph;
+            f.hc2 = (float)h.hc2;
+            f.hs2m2 = (float)(-2.0 * h.hs2);
+            f.hs2h = (float)(0.5 * h.hs2);
+            const double R = kFp32NearScaleSis * std::fabs(b0);
+            f.near2 = (float)(R * R);
+        } else {
+            f.near2 = 3.0e38f;  // type 14 stays FP64 (a constant per halo)
+        }
     }
     // runs: equal type (and, for the Kmap, equal dlsds); the scan never reads past n - 1
     int s = 0;

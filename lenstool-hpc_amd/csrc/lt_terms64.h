@@ -1,0 +1,329 @@
+// lt_terms64.h -- FP64 per-(halo, pixel) terms, map epilogues and launch geometry shared by the
+// FP64 kernels (lt_kernels64.hip) and the FP32-compensated kernels (lt_kernels32.hip), which fall
+// back to these terms near halo cores.
+//
+// Formulas (restated from the mathematics; reference equivalents cited):
+//   SIS gradient  src/gradient.cpp:260-299      PIEMD gradient  src/gradient.hpp:64-76,
+//   dPIE gradient src/gradient.cpp:744-809      PIEMD Hessian   src/gradient2.cpp:54-82, 92-205
+//   potential     src/potential_GPU.cu:47-116   epilogues       src/grid_map_*_GPU.cu
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "lenstool_hip/lt_status.h"
+#include "lt_device.h"
+#include "lt_math.h"
+
+namespace lt {
+
+// ------------------------------------------------------------------------------------------------
+// per-halo Hessian contributions, accumulated in the lab frame as
+//   T = a + c (trace, rotation invariant), D = a - c, B = b.
+// Halo frame -> lab frame: D' = cos2t D - 2 sin2t B, B' = sin2t D / 2 + cos2t B; the halo amplitude is
+// folded into hc2 = amph cos2t and hs2 = amph sin2t.
+// ------------------------------------------------------------------------------------------------
+
+struct PixelFrame {
+    double x, y, x2, y2;
+};
+
+__device__ __forceinline__ PixelFrame halo_frame(const HaloRec& h, double px, double py)
+{
+    PixelFrame f;
+    double tx = px - h.xh;
+    double ty = py - h.yh;
+    f.x = __builtin_fma(tx, h.cs, ty * h.sn);
+    f.y = __builtin_fma(ty, h.cs, -(tx * h.sn));
+    f.x2 = f.x * f.x;
+    f.y2 = f.y * f.y;
+    return f;
+}
+
+// One I_{0.5} term of the PIEMD Hessian in the halo frame (Kassiola & Kovner 1993), split into
+// trace / difference / off-diagonal parts. The den2 pieces cancel in the trace.
+struct PiemdH {
+    double T, D, B;
+};
+
+__device__ __forceinline__ PiemdH piemd_hess_term(const HaloRec& h, const PixelFrame& f, double rem2,
+                                                  double yi, double cxx, double X1, double Y6,
+                                                  double Y5, double XY3, double r2, double e2r)
+{
+    double q = r2 + rem2;
+    double iw = rsq64(q);
+    double w = q * iw;
+    double t = __builtin_fma(h.e2, w, -yi);
+    double den1 = __builtin_fma(t, t, cxx);
+    double n2 = e2r - f.y;
+    double den2 = __builtin_fma(n2, n2, f.x2);
+    double ip = rcp64(den1 * den2);
+    double id1 = den2 * ip;
+    double id2 = den1 * ip;
+    double bxx = __builtin_fma(X1, iw, __builtin_fma(-h.hk2, w, f.y));
+    double byy = __builtin_fma(h.hk4, w, __builtin_fma(Y6, iw, -Y5));
+    double bxy = __builtin_fma(XY3, iw, -f.x);
+    PiemdH o;
+    o.T = (bxx + byy) * id1;
+    o.D = __builtin_fma(bxx - byy, id1, (n2 + n2) * id2);
+    o.B = __builtin_fma(bxy, id1, f.x * id2);
+    return o;
+}
+
+template <bool CUT>
+__device__ __forceinline__ void hess_piemd(const HaloRec& h, double px, double py, double& aT, double& aD,
+                                           double& aB)
+{
+    PixelFrame f = halo_frame(h, px, py);
+    double rem2 = __builtin_fma(f.x2, h.kx, f.y2 * h.ky);
+    double yi = f.y * h.icx1;
+    double cxx = h.cx1sq * f.x2;
+    double X1 = h.hk1 * f.x2;
+    double Y6 = h.hk6 * f.y2;
+    double Y5 = h.hk5 * f.y;
+    double XY3 = h.hk3 * f.x * f.y;
+    PiemdH a = piemd_hess_term(h, f, rem2, yi, cxx, X1, Y6, Y5, XY3, h.ra2, h.e2ra);
+    double dT = a.T, dD = a.D, dB = a.B;
+    if (CUT) {
+        PiemdH b = piemd_hess_term(h, f, rem2, yi, cxx, X1, Y6, Y5, XY3, h.rb2, h.e2rb);
+        dT -= b.T;
+        dD -= b.D;
+        dB -= b.B;
+    }
+    aT = __builtin_fma(h.amph, dT, aT);
+    aD = __builtin_fma(h.hc2, dD, aD);
+    aD = __builtin_fma(-2.0 * h.hs2, dB, aD);
+    aB = __builtin_fma(0.5 * h.hs2, dD, aB);
+    aB = __builtin_fma(h.hc2, dB, aB);
+}
+
+// SIS / SIE: phi = b0 R, R^2 = (1-e) x^2 + (1+e) y^2; halo-frame Hessian
+// amph/R^3 (y^2, -xy, x^2) with amph = b0 (1-e)(1+e).
+__device__ __forceinline__ void hess_sis(const HaloRec& h, double px, double py, double& aT, double& aD,
+                                         double& aB)
+{
+    PixelFrame f = halo_frame(h, px, py);
+    double R2 = __builtin_fma(h.sa, f.x2, h.sb * f.y2);
+    double iR = rsq64(R2);
+    double g = iR * iR * iR;
+    double dyx = f.y2 - f.x2;
+    double xy = f.x * f.y;
+    aT = __builtin_fma(g * h.amph, f.x2 + f.y2, aT);
+    aD = __builtin_fma(g, __builtin_fma(h.hc2, dyx, 2.0 * h.hs2 * xy), aD);
+    aB = __builtin_fma(g, __builtin_fma(0.5 * h.hs2, dyx, -(h.hc2 * xy)), aB);
+}
+
+// ------------------------------------------------------------------------------------------------
+// gradient terms, accumulated in the lab frame
+// ------------------------------------------------------------------------------------------------
+
+// N conj(D) of the I_{0.5} ratio Z = N/D (N = cx1 x + i t, D = x + i n2); its real part is
+// non-negative everywhere, so arg Z lies in [-pi/2, pi/2].
+struct NconjD {
+    double re, im, den1, den2;
+};
+
+__device__ __forceinline__ NconjD piemd_ratio(const HaloRec& h, const PixelFrame& f, double rem2, double yi,
+                                              double cxx, double r2, double e2r)
+{
+    double q = r2 + rem2;
+    double w = q * rsq64(q);
+    double t = __builtin_fma(h.e2, w, -yi);
+    double n2 = e2r - f.y;
+    NconjD o;
+    o.re = __builtin_fma(t, n2, h.cx1 * f.x2);
+    o.im = f.x * __builtin_fma(-h.cx1, n2, t);
+    o.den1 = __builtin_fma(t, t, cxx);
+    o.den2 = __builtin_fma(n2, n2, f.x2);
+    return o;
+}
+
+// PIEMD (CUT = false) or dPIE (CUT = true): (ampg arg, -ampg ln|.|) of Z(rc) or Z(rc)/Z(rcut).
+// For the dPIE the two logarithms and the two arguments are merged into one of each:
+// ln|Za| - ln|Zb| = ln(den1a den2b / (den2a den1b)) / 2, arg Za - arg Zb = arg(Na conj(Da) Nb' Db)
+// (both arguments in [-pi/2, pi/2], so the difference needs no unwrapping).
+template <bool CUT>
+__device__ __forceinline__ void grad_piemd(const HaloRec& h, double px, double py, double& gx, double& gy)
+{
+    PixelFrame f = halo_frame(h, px, py);
+    double rem2 = __builtin_fma(f.x2, h.kx, f.y2 * h.ky);
+    double yi = f.y * h.icx1;
+    double cxx = h.cx1sq * f.x2;
+    NconjD a = piemd_ratio(h, f, rem2, yi, cxx, h.ra2, h.e2ra);
+    double pre, pim, num, den;
+    if (CUT) {
+        NconjD b = piemd_ratio(h, f, rem2, yi, cxx, h.rb2, h.e2rb);
+        pre = __builtin_fma(a.re, b.re, a.im * b.im);
+        pim = __builtin_fma(a.im, b.re, -(a.re * b.im));
+        num = a.den1 * b.den2;
+        den = a.den2 * b.den1;
+    } else {
+        pre = a.re;
+        pim = a.im;
+        num = a.den1;
+        den = a.den2;
+    }
+    double arg = atan2(pim, pre);
+    double lnr = 0.5 * log(num * rcp64(den));
+    double re = h.ampg * arg;
+    double im = -h.ampg * lnr;
+    gx += __builtin_fma(re, h.cs, -(im * h.sn));
+    gy += __builtin_fma(im, h.cs, re * h.sn);
+}
+
+__device__ __forceinline__ void grad_sis(const HaloRec& h, double px, double py, double& gx, double& gy)
+{
+    PixelFrame f = halo_frame(h, px, py);
+    double R2 = __builtin_fma(h.sa, f.x2, h.sb * f.y2);
+    double s = h.ampg * rsq64(R2);
+    double rx = h.sa * f.x * s;
+    double ry = h.sb * f.y * s;
+    gx += __builtin_fma(rx, h.cs, -(ry * h.sn));
+    gy += __builtin_fma(ry, h.cs, rx * h.sn);
+}
+
+// ------------------------------------------------------------------------------------------------
+// potential (complex I_{0.5} potential of Kassiola & Kovner, src/potential_GPU.cu:47-72)
+// ------------------------------------------------------------------------------------------------
+struct cplx {
+    double re, im;
+};
+__device__ __forceinline__ cplx cmul(cplx a, cplx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ __forceinline__ cplx cdiv(cplx a, cplx b)
+{
+    double n = 1.0 / (b.re * b.re + b.im * b.im);
+    return {(a.re * b.re + a.im * b.im) * n, (a.im * b.re - a.re * b.im) * n};
+}
+__device__ __forceinline__ cplx ccosh(cplx a)
+{
+    double ch = cosh(a.re), sh = sinh(a.re), s, c;
+    sincos(a.im, &s, &c);
+    return {ch * c, sh * s};
+}
+__device__ __forceinline__ cplx csinh(cplx a)
+{
+    double ch = cosh(a.re), sh = sinh(a.re), s, c;
+    sincos(a.im, &s, &c);
+    return {sh * c, ch * s};
+}
+__device__ __forceinline__ cplx clog(cplx a) { return {0.5 * log(a.re * a.re + a.im * a.im), atan2(a.im, a.re)}; }
+
+__device__ inline double pi05(double x, double y, double eps, double rc, double b0)
+{
+    double sqe = sqrt(eps);
+    double ci = .5 * (1. - eps * eps) / sqe;
+    double rem2 = x * x / ((1. + eps) * (1. + eps)) + y * y / ((1. - eps) * (1. - eps));
+    double e1 = 2. * sqe / (1 - eps);
+    double e2 = 2. * sqe / (1 + eps);
+    double z = sqrt(x * x + y * y);
+    cplx eta = {-.5 * asinh(e1 * y / z), .5 * asin(e2 * x / z)};
+    double srem = sqrt(rem2);
+    cplx zeta = {0.5 * log((srem + sqrt(rc * rc + rem2)) / rc), 0.};
+    cplx b1 = ccosh({eta.re + zeta.re, eta.im});
+    cplx b2 = ccosh({eta.re - zeta.re, eta.im});
+    cplx ce = ccosh(eta);
+    cplx a1 = clog(cdiv(cmul(ce, ce), cmul(b1, b2)));
+    cplx a2 = clog(cdiv(b1, b2));
+    cplx c1 = cmul(csinh({2. * eta.re, 2. * eta.im}), a1);
+    cplx c2 = cmul(csinh({2. * zeta.re, 0.}), a2);
+    return b0 * ci * rc / srem * ((c1.im + c2.im) * x - (c1.re + c2.re) * y);
+}
+
+__device__ __forceinline__ double pot_halo(const HaloRec& h, int type, double px, double py)
+{
+    PixelFrame f = halo_frame(h, px, py);
+    if (type == TYPE_SIS) return h.b0 * sqrt(__builtin_fma(h.sa, f.x2, h.sb * f.y2));
+    double pa = pi05(f.x, f.y, h.eps, h.rc, h.b0);
+    if (type == TYPE_PIEMD) return pa;
+    double ps = pi05(f.x, f.y, h.eps, h.rcut, h.b0);
+    return (h.rcut / (h.rcut - h.rc)) * (pa - ps);
+}
+
+// ------------------------------------------------------------------------------------------------
+// epilogues (src/grid_map_ampli_GPU.cu:329-572, grid_map_shear_GPU.cu:208-342,
+// grid_map_mass_GPU.cu:307-322); T, D, B are the Hessian sums a+c, a-c, b.
+// The map kind is a template parameter, and within a kind every mode's value is formed and one of
+// them selected, so each kernel has a single store per output plane with a fixed base pointer.
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ void eigen(double A, double B, double C, double& lam, double& mu)
+{
+    double e = sqrt((A - C) * (A - C) + 4. * B * B);
+    lam = .5 * (A + C + e);
+    mu = .5 * (A + C - e);
+}
+
+template <int KIND>
+__device__ __forceinline__ void hess_epilogue(const OutDesc& o, int64_t idx, double T, double D, double B)
+{
+    const double a = 0.5 * (T + D), c = 0.5 * (T - D), b = B;
+    const double r = o.r;
+    if constexpr (KIND == LT_MAP_HESSIAN) {
+        if (o.p0) o.p0[idx] = a;
+        if (o.p1) o.p1[idx] = b;
+        if (o.p2) o.p2[idx] = c;
+        if (o.p3) o.p3[idx] = b;
+    } else if constexpr (KIND == LT_MAP_FIELDS) {
+        double kappa = 0.5 * T * r, g1 = 0.5 * D * r, g2 = B * r;
+        double omk = 1. - kappa;
+        double det = __builtin_fma(omk, omk, -__builtin_fma(g1, g1, g2 * g2));
+        if (o.p0) o.p0[idx] = kappa;
+        if (o.p1) o.p1[idx] = g1;
+        if (o.p2) o.p2[idx] = g2;
+        if (o.p3) o.p3[idx] = det;
+        if (o.p4) o.p4[idx] = 1. / det;
+    } else if constexpr (KIND == LT_MAP_AMPLI) {
+        double lam, mu;
+        eigen(1. - a * r, -b * r, 1. - c * r, lam, mu);
+        const double lm = lam * mu;
+        const double kappa = (a + c) * r / 2., ga1 = (a - c) * r / 2., ga2 = b * r;
+        const double gp = sqrt(ga1 * ga1 + ga2 * ga2) / (1 - kappa);
+        const double h1 = (a - c) / 2.;
+        const int m = o.mode;
+        double v = sqrt(h1 * h1 + b * b);                                       // 6: gamma, no dlsds
+        v = m == 5 ? (a + c) / 2. : v;                                          // 5: kappa, no dlsds
+        v = m == 4 ? (1 - kappa) * (1 + gp * gp) / (1 - gp * gp) : v;           // 4
+        v = m == 3 ? -2.5 * log10(fabs(lm)) : v;                                // 3
+        v = m == 2 ? 1. / fabs(lm) : v;                                         // 2
+        v = m == 1 ? 1. / lm : v;                                               // 1
+        o.p0[idx] = v;
+    } else if constexpr (KIND == LT_MAP_SHEAR) {
+        double lam, mu;
+        eigen(1. - a * r, -b * r, 1. - c * r, lam, mu);
+        const double q = lam / mu;
+        const int m = o.mode;
+        double v = b * r;                                                       // 4: gamma2
+        v = m == 3 ? (a - c) * r / 2. : v;                                      // 3: gamma1
+        v = m == 2 ? fabs((q * q - 1.) / (q * q + 1.)) : v;                     // 2
+        v = m == 1 ? (lam - mu) / 2. : v;                                       // 1
+        o.p0[idx] = v;
+    } else {  // LT_MAP_MASS
+        o.p0[idx] = (a + c) * 0.5 * o.mult;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// kernels
+// ------------------------------------------------------------------------------------------------
+constexpr int WAVE = 64;
+constexpr int ROWS = 4;
+
+// Sample s of a batch (blockIdx.z) uses its own halo records and output planes.
+__device__ __forceinline__ const HaloRec* select_sample(const GridDesc& g, const HaloRec* halos, OutDesc& o)
+{
+    const int64_t s = blockIdx.z;
+    const int64_t off = s * g.plane_stride;
+    if (o.p0) o.p0 += off;
+    if (o.p1) o.p1 += off;
+    if (o.p2) o.p2 += off;
+    if (o.p3) o.p3 += off;
+    if (o.p4) o.p4 += off;
+    return halos + s * g.halo_stride;
+}
+
+// Launch geometry: a 64 x 4 work-group, PX pixels per thread along a row, blockIdx.z = sample.
+inline dim3 grid_for(const GridDesc& g, int px)
+{
+    return dim3((unsigned)((g.nx + WAVE * px - 1) / (WAVE * px)), (unsigned)((g.ny + ROWS - 1) / ROWS),
+                (unsigned)(g.nsamples > 0 ? g.nsamples : 1));
+}
+
+}  // namespace lt

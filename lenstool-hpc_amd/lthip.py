@@ -78,6 +78,14 @@ SIGNATURES = {
     "lt_cosmo_dlsds": ([C.c_double, C.c_double, _cp], C.c_double),
     "lt_model_create": ([C.POINTER(C.c_void_p), _soa, C.c_int, C.c_int], C.c_int),
     "lt_model_update": ([C.c_void_p, _soa, C.c_int, C.c_void_p], C.c_int),
+    "lt_model_create_batch": ([C.POINTER(C.c_void_p), _soa, C.c_int, C.c_int, C.c_int], C.c_int),
+    "lt_model_update_batch": ([C.c_void_p, _soa, C.c_int, C.c_void_p], C.c_int),
+    "lt_model_stage": ([C.c_void_p, _soa, C.c_int], C.c_int),
+    "lt_model_upload": ([C.c_void_p, C.c_void_p], C.c_int),
+    "lt_graph_begin": ([C.c_void_p], C.c_int),
+    "lt_graph_end": ([C.c_void_p, C.POINTER(C.c_void_p)], C.c_int),
+    "lt_graph_launch": ([C.c_void_p, C.c_void_p], C.c_int),
+    "lt_graph_destroy": ([C.c_void_p], C.c_int),
     "lt_model_destroy": ([C.c_void_p], C.c_int),
     "lt_model_bytes": ([C.c_void_p, C.POINTER(C.c_longlong)], C.c_int),
     "lt_map_device": ([C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -316,6 +324,33 @@ class DeviceArray:
             pass
 
 
+class Graph:
+    """Launches captured on a stream between begin() and end(), replayed by launch()."""
+
+    def __init__(self, stream):
+        self.h = C.c_void_p()
+        self.stream = stream
+        check(lib().lt_graph_begin(C.c_void_p(stream)))
+
+    def end(self):
+        check(lib().lt_graph_end(C.c_void_p(self.stream), C.byref(self.h)))
+        return self
+
+    def launch(self, stream=None):
+        check(lib().lt_graph_launch(self.h, C.c_void_p(self.stream if stream is None else stream)))
+
+    def close(self):
+        if self.h:
+            lib().lt_graph_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def stream_create() -> int:
     s = C.c_void_p()
     check(lib().lt_stream_create(C.byref(s)))
@@ -359,10 +394,31 @@ class Event:
 # ------------------------------------------------------------------------------------------------
 
 class Model:
-    def __init__(self, lens: M.LensModel, use=MAP_HESSIAN):
+    """A resident lens model, or a batch of samples of one (lenses = list of LensModel)."""
+
+    def __init__(self, lens, use=MAP_HESSIAN):
         self.h = C.c_void_p()
         self.lens = lens
-        check(lib().lt_model_create(C.byref(self.h), C.byref(lens.soa()), lens.n, use))
+        if isinstance(lens, (list, tuple)):
+            arr = (M.LtPotentialSOA * len(lens))(*[l.soa() for l in lens])
+            check(lib().lt_model_create_batch(C.byref(self.h), arr, len(lens), lens[0].n, use))
+            self.nsamples = len(lens)
+        else:
+            check(lib().lt_model_create(C.byref(self.h), C.byref(lens.soa()), lens.n, use))
+            self.nsamples = 1
+
+    def update_batch(self, lenses, stream=0):
+        arr = (M.LtPotentialSOA * len(lenses))(*[l.soa() for l in lenses])
+        check(lib().lt_model_update_batch(self.h, arr, len(lenses), C.c_void_p(stream)))
+        self.lens = list(lenses)
+
+    def stage(self, lenses):
+        lenses = lenses if isinstance(lenses, (list, tuple)) else [lenses]
+        arr = (M.LtPotentialSOA * len(lenses))(*[l.soa() for l in lenses])
+        check(lib().lt_model_stage(self.h, arr, len(lenses)))
+
+    def upload(self, stream=0):
+        check(lib().lt_model_upload(self.h, C.c_void_p(stream)))
 
     def update(self, lens: M.LensModel, stream=0):
         self.lens = lens

@@ -35,3 +35,8 @@ def hessian_reference(O, lens, grid, istart=0, jstart=0, kmap=False):
         ch = ch * w
     ta, tb, tc = REL * sa + ch, REL * sb + ch, REL * sc + ch
     return (ea, eb, ec), (ta, tb, tc)
+
+
+def print_case(name, lens, grid):
+    """Name the case on stdout before a GPU step, so that a failing run says which one it was."""
+    print(f"[case] {name}: {lens.n} halos, {grid.nx}x{grid.ny}", flush=True)

@@ -12,7 +12,7 @@ import pytest
 
 import ltmodel as M
 from conftest import GOLDEN
-from parity import REL, assert_close, hessian_reference
+from parity import REL, assert_close, hessian_reference, print_case
 
 pytestmark = pytest.mark.gpu
 
@@ -39,10 +39,6 @@ def mixed_lens(seed=7, n81=5, n8=3, n5=4, n14=0):
                 h["e"] = float(rng.uniform(0.01, 0.1))
                 h["epot"] = h["e"]
     return M.LensModel.from_halos(halos)
-
-
-def print_case(name, lens, grid):
-    print(f"[case] {name}: {lens.n} halos, {grid.nx}x{grid.ny}", flush=True)
 
 
 # ------------------------------------------------------------------------------------------------
