@@ -102,8 +102,190 @@ def cpu_baseline(lens, grid, budget_s=6.0):
                       f"grid, rows 2048.., {dt:.1f} s, OMP threads = {threads}"}
 
 
+def bench_bayes(a):
+    """Batched Bayes maps: nsamples perturbations of the 1 dPIE + 10 SIS cluster lens (cluster x, y
+    by N(0, 0.5"), e, theta, rcore, sigma by 5%, galaxy sigma by a common 5% factor), each a 2048^2
+    amplification map (mode 1, mu), all samples in one kernel launch inside a captured graph that
+    also uploads the staged parameters. A step = stage a new sample set on the host + replay."""
+    S, n = a.samples, 2048
+    base = M.config1_lens(seed=1)
+    sets = [M.perturbed_samples(base, S, seed=100 + k) for k in range(2)]
+    grid = M.Grid(M.grid_param(-100.0, 100.0, -100.0, 100.0), n, n)
+    r = lthip.cosmo_dlsds(0.4, 2.0, M.flat_lcdm(70.0, 0.3))
+    model = lthip.Model(sets[0], lthip.MAP_HESSIAN)
+    out = lthip.DeviceArray(S * grid.npix)
+    stream = lthip.stream_create()
+    res = {}
+    keep = {}
+    for name, prec in (("fp64", lthip.PREC_FP64), ("fp32c", lthip.PREC_FP32C)):
+        g = lthip.Graph(stream)
+        model.upload(stream)
+        model.map(lthip.MAP_AMPLI, 1, [out.ptr], grid, r=r, precision=prec, stream=stream)
+        g.end()
+        for k in range(a.warmup):
+            model.stage(sets[k % 2])
+            g.launch()
+            lthip.sync(stream)
+        ev = [lthip.Event() for _ in range(a.steps + 1)]
+        per = []
+        t0 = time.perf_counter()
+        for k in range(a.steps):
+            model.stage(sets[k % 2])
+            ev[k].record(stream)
+            g.launch()
+            ev[k + 1].record(stream)
+            lthip.sync(stream)
+            per.append(ev[k].elapsed_ms(ev[k + 1]))
+        wall = time.perf_counter() - t0
+        work = S * grid.npix * base.n
+        res[name] = {"value": a.steps * work / wall, "unit": "halo·px/s", "ms_per_step": wall / a.steps * 1e3,
+                     "device_ms_p50": percentile(per, 50), "device_ms_p99": percentile(per, 99),
+                     "device_halo_px_per_s": work / (np.mean(per) * 1e-3)}
+        # both precisions end on the same sample set: keep four samples' mu for the error check
+        keep[name] = [out.download_range(s * grid.npix, grid.npix) for s in (0, S // 3, 2 * S // 3, S - 1)]
+        g.close()
+    det_err = max(float(np.abs(1 / x - 1 / y).max()) for x, y in zip(keep["fp32c"], keep["fp64"]))
+    res["fp32c"]["max_abs_err_detA_vs_fp64"] = det_err
+    return {"metric": "halo·pixel evals/s, batched Bayes amplification maps (one graph launch per step)",
+            "value": res["fp64"]["value"], "unit": "halo·px/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+            "ms_per_step": res["fp64"]["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"{S} Bayes samples x 2048^2 mu maps, 1 dPIE + 10 SIS", "samples": S,
+                       "pixels_per_sample": grid.npix, "halos": base.n},
+            "fp64": res["fp64"], "fp32c": res["fp32c"]}
+
+
+def bench_chi(a):
+    """Delensing + chi^2 batched over lens samples: the 1 dPIE + 10 SIS lens with 40 families of 3
+    images (sources uniform in [-20, 20]^2 arcsec, z_s ~ U[1, 6]; images = barycentres of this
+    library's own delensing of the true model on a 2048^2 grid + N(0, 0.1") noise), delensing grid
+    512^2, samples perturbed as for the Bayes maps. A step = one ChiBatch run over all samples."""
+    rng = np.random.default_rng(4)
+    lens = M.config1_lens(seed=1)
+    cosmo = M.flat_lcdm(70.0, 0.3)
+    frame = M.grid_param(-100.0, 100.0, -100.0, 100.0)
+    fine = M.Grid(frame, 2048, 2048)
+    gx, gy = lthip.gradient_grid(lens, fine)
+    fams = []
+    while len(fams) < 40:
+        src = M.galaxies_array(16)
+        src["cx"] = rng.uniform(-20, 20, 16)
+        src["cy"] = rng.uniform(-20, 20, 16)
+        zs = rng.uniform(1.0, 6.0, 16)
+        src["dr"] = [lthip.cosmo_dlsds(0.4, z, cosmo) for z in zs]
+        _, pos, loc, _ = lthip.delense_barycenter(16, fine.nx, frame, src, gx.ravel(), gy.ravel())
+        for s in range(16):
+            if loc[s] >= 3 and len(fams) < 40:
+                fams.append((pos[s, :3].copy(), src["dr"][s], zs[s]))
+    nimages = np.full(40, 3, dtype=np.int32)
+    images = M.galaxies_array(120)
+    for f, (p, dr, z) in enumerate(fams):
+        for k in range(3):
+            images[3 * f + k]["cx"] = p[k, 0] + rng.normal(0, 0.1)
+            images[3 * f + k]["cy"] = p[k, 1] + rng.normal(0, 0.1)
+            images[3 * f + k]["dr"] = dr
+            images[3 * f + k]["redshift"] = z
+    S = a.samples
+    samples = M.perturbed_samples(lens, S, seed=5)
+    b = lthip.ChiBatch(S, lens.type, 512, frame, nimages, images)
+    for _ in range(a.warmup):
+        b.run(samples)
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        chi, found, status = b.run(samples)
+    wall = time.perf_counter() - t0
+    b.close()
+    return {"metric": "chi^2 samples/s (gradient grid + barycentres + delensing + chi^2 per sample)",
+            "value": a.steps * S / wall, "unit": "samples/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+            "ms_per_step": wall / a.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"{S} samples, 40 families x 3 images, delensing grid 512^2, 1 dPIE + 10 SIS",
+                       "samples": S},
+            "gradient_halo_px_per_s": a.steps * S * 512 * 512 * lens.n / wall,
+            "chi_p50": float(np.median(chi)), "found_mean": float(found.mean()),
+            "overflow_samples": int((status != 0).sum())}
+
+
+def bench_serve(a):
+    """Map-serving loop: Poisson arrivals at each rate of --rates; grid n = 2^U[8,13] rounded to a
+    multiple of 64; a pool of 32 resident models drawn like the cluster lens with
+    H = floor(10^U[1, log10 2000]) halos; map kind uniform over {gradient (dpl), ampli (mu), shear,
+    mass (Kmap), pot}; precision uniform over {FP32C, FP64}. Reports per rate the throughput and the
+    p50/p99 latency from submission to result ready on the device and to result copied to the host."""
+    import threading
+    rng = np.random.default_rng(9)
+    pool = []
+    for k in range(32):
+        H = int(10 ** rng.uniform(1.0, np.log10(2000.0)))
+        L = M.config2_lens(seed=1000 + k, ngal=max(H - 3, 0))
+        L.dlsds[:] = 0.7
+        L._soa = None
+        pool.append(L)
+    r = lthip.cosmo_dlsds(0.4, 2.0, M.flat_lcdm(70.0, 0.3))
+    srv = lthip.Server(nslots=a.slots, hbm_budget_bytes=a.budget_gb << 30)
+    mids = [srv.add_model(L) for L in pool]
+    kinds = [(lthip.MAP_DPL, 1), (lthip.MAP_AMPLI, 1), (lthip.MAP_SHEAR, 1), (lthip.MAP_MASS, 1), (lthip.MAP_POT, 1)]
+
+    def make_req():
+        n = int(round(2 ** rng.uniform(8, 13) / 64.0)) * 64
+        n = min(max(n, 256), 8192)
+        k, m = kinds[rng.integers(len(kinds))]
+        mi = int(rng.integers(len(pool)))
+        grid = M.Grid(M.grid_param(-150.0, 150.0, -150.0, 150.0), n, n)
+        return mi, k, m, grid, int(rng.integers(2))
+
+    rows = []
+    for rate in a.rates:
+        reqs = [make_req() for _ in range(a.requests)]
+        gaps = rng.exponential(1.0 / rate, len(reqs))
+        srv.reset_stats()
+        tickets = [None] * len(reqs)
+        ready = threading.Semaphore(0)
+
+        def producer():
+            t_next = time.perf_counter()
+            for i, (mi, k, m, grid, p) in enumerate(reqs):
+                t_next += gaps[i]
+                d = t_next - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                tickets[i] = srv.submit(mids[mi], k, m, grid, precision=p, r=r, mult=1.0)
+                ready.release()
+
+        th = threading.Thread(target=producer)
+        t0 = time.perf_counter()
+        th.start()
+        for i in range(len(reqs)):
+            ready.acquire()
+            _, k, _, grid, _ = reqs[i]
+            srv.fetch(tickets[i], k, grid.npix)
+        th.join()
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+        rows.append({"rate_per_s": rate, "requests": len(reqs), "wall_s": wall,
+                     "halo_px_per_s": st["halo_px"] / wall, "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
+                     "p50_fetch_ms": st["p50_fetch_ms"], "p99_fetch_ms": st["p99_fetch_ms"],
+                     "bytes_peak": st["bytes_peak"]})
+        print(f"[serve] rate {rate}/s: {json.dumps(rows[-1])}", file=sys.stderr, flush=True)
+    srv.close()
+    best = max(rows, key=lambda x: x["halo_px_per_s"])
+    return {"metric": "map-serving throughput (halo·px/s) and p50/p99 latency under Poisson load",
+            "value": best["halo_px_per_s"], "unit": "halo·px/s", "n_gpus": 1, "steps": a.requests,
+            "warmup": 0, "ms_per_step": best["wall_s"] / best["requests"] * 1e3, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64|f32+f64 (per request)", "data": "synthetic",
+            "config": {"workload": "Poisson map requests, n in 256..8192, H in 10..2000, 5 map kinds, "
+                                   "FP32C or FP64", "slots": a.slots, "hbm_budget_gb": a.budget_gb},
+            "latency_ms": {"p50": best["p50_ms"], "p99": best["p99_ms"]}, "sweep": rows}
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", choices=("cluster", "bayes", "chi", "serve"), default="cluster")
+    ap.add_argument("--samples", type=int, default=0, help="bayes: 256, chi: 1024 by default")
+    ap.add_argument("--rates", type=float, nargs="*", default=[2.0, 8.0, 32.0])
+    ap.add_argument("--requests", type=int, default=40)
+    ap.add_argument("--slots", type=int, default=8)
+    ap.add_argument("--budget-gb", type=int, default=64)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
@@ -118,6 +300,12 @@ def main():
               "multi-GPU code in this build", file=sys.stderr)
         sys.exit(2)
     lthip.check(lthip.lib().lt_init(0))
+    if a.workload != "cluster":
+        if not a.samples:
+            a.samples = 256 if a.workload == "bayes" else 1024
+        fn = {"bayes": bench_bayes, "chi": bench_chi, "serve": bench_serve}[a.workload]
+        print(json.dumps(fn(a)), flush=True)
+        return
 
     lens = M.config2_lens(seed=2)
     grid = M.config2_grid(4096)

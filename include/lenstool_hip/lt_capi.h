@@ -159,6 +159,48 @@ int lt_graph_end(void* stream, lt_graph* out);
 int lt_graph_launch(lt_graph graph, void* stream);
 int lt_graph_destroy(lt_graph graph);
 
+/* ---- map-serving loop ---------------------------------------------------------------------------
+ * The reference has no counterpart (its map drivers, utils/exec/main.cpp:397-545, run one
+ * synchronous map after another). A server keeps lens models resident on the calling thread's GPU,
+ * runs requests on nslots streams and admits queued requests as slots and HBM (hbm_budget_bytes,
+ * counting the output planes of admitted requests and its cache of freed planes) become free; work
+ * of different requests overlaps on the device. Results stay in device memory until fetched. */
+typedef struct lt_server_s* lt_server;
+typedef struct lt_request {
+    int model;            /* id from lt_server_add_model                                          */
+    int kind, mode;       /* LT_MAP_* and mode, as lt_map_device                                  */
+    int precision;        /* LT_PREC_FP64 or LT_PREC_FP32C                                        */
+    lt_grid_desc grid;    /* pitch is ignored (planes are nx * ny)                                */
+    double r, mult;       /* map-formula scalars, as lt_map_device                                */
+} lt_request;
+typedef struct lt_request_times {
+    double submit_ms, start_ms, done_ms, fetched_ms;  /* since server creation; done = observed on device */
+} lt_request_times;
+typedef struct lt_server_stats {
+    long long submitted, completed, failed, queued, running;
+    double p50_ms, p99_ms;              /* submit -> result ready on the device                   */
+    double p50_fetch_ms, p99_fetch_ms;  /* submit -> result copied to the host by lt_server_fetch */
+    double halo_px;                     /* sum over completed requests of halos x pixels          */
+    double elapsed_ms;                  /* since creation or the last reset                       */
+    long long bytes_in_use, bytes_peak;
+} lt_server_stats;
+int lt_server_create(lt_server* out, int nslots, long long hbm_budget_bytes);
+/* Pack the model for every map kind it supports (gradient, Hessian, Kmap when dlsds is given,
+ * potential); *model_id indexes it in requests. */
+int lt_server_add_model(lt_server server, const lt_potential_soa* lens, int nhalos, int* model_id);
+/* Queue a request (non-blocking). LT_ERR_CAPACITY if its planes alone exceed the budget. */
+int lt_server_submit(lt_server server, const lt_request* req, long long* ticket);
+/* Wait for a request (timeout_ms < 0: forever); LT_ERR_BUSY on timeout, the request's status if it failed. */
+int lt_server_wait(lt_server server, long long ticket, double timeout_ms);
+/* Wait, copy the request's planes to the given host arrays (NULL = skip a plane), release the
+ * ticket and its device memory. Plane order as lt_map_device. */
+int lt_server_fetch(lt_server server, long long ticket, double* out0, double* out1, double* out2, double* out3,
+                    double* out4, lt_request_times* times);
+int lt_server_get_stats(lt_server server, lt_server_stats* stats);
+int lt_server_reset_stats(lt_server server);
+/* Stops the dispatcher, waits for the device and frees everything (unfetched results included). */
+int lt_server_destroy(lt_server server);
+
 /* ---- point evaluations, delensing and chi^2 ------------------------------------------------------ */
 
 /* Gradient at npts points (host arrays). The batched counterpart of the reference's point evaluator

@@ -34,6 +34,23 @@ class LtGridDesc(C.Structure):
                 ("pitch", C.c_longlong)]
 
 
+class LtRequest(C.Structure):
+    _fields_ = [("model", C.c_int), ("kind", C.c_int), ("mode", C.c_int), ("precision", C.c_int),
+                ("grid", LtGridDesc), ("r", C.c_double), ("mult", C.c_double)]
+
+
+class LtRequestTimes(C.Structure):
+    _fields_ = [("submit_ms", C.c_double), ("start_ms", C.c_double), ("done_ms", C.c_double),
+                ("fetched_ms", C.c_double)]
+
+
+class LtServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_longlong), ("completed", C.c_longlong), ("failed", C.c_longlong),
+                ("queued", C.c_longlong), ("running", C.c_longlong), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
+                ("p50_fetch_ms", C.c_double), ("p99_fetch_ms", C.c_double), ("halo_px", C.c_double),
+                ("elapsed_ms", C.c_double), ("bytes_in_use", C.c_longlong), ("bytes_peak", C.c_longlong)]
+
+
 class LtError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__(f"lenstool_hip status {status}: {msg}")
@@ -82,6 +99,14 @@ SIGNATURES = {
     "lt_model_update_batch": ([C.c_void_p, _soa, C.c_int, C.c_void_p], C.c_int),
     "lt_model_stage": ([C.c_void_p, _soa, C.c_int], C.c_int),
     "lt_model_upload": ([C.c_void_p, C.c_void_p], C.c_int),
+    "lt_server_create": ([C.POINTER(C.c_void_p), C.c_int, C.c_longlong], C.c_int),
+    "lt_server_add_model": ([C.c_void_p, _soa, C.c_int, C.POINTER(C.c_int)], C.c_int),
+    "lt_server_submit": ([C.c_void_p, C.POINTER(LtRequest), C.POINTER(C.c_longlong)], C.c_int),
+    "lt_server_wait": ([C.c_void_p, C.c_longlong, C.c_double], C.c_int),
+    "lt_server_fetch": ([C.c_void_p, C.c_longlong] + [C.c_void_p] * 5 + [C.POINTER(LtRequestTimes)], C.c_int),
+    "lt_server_get_stats": ([C.c_void_p, C.POINTER(LtServerStats)], C.c_int),
+    "lt_server_reset_stats": ([C.c_void_p], C.c_int),
+    "lt_server_destroy": ([C.c_void_p], C.c_int),
     "lt_graph_begin": ([C.c_void_p], C.c_int),
     "lt_graph_end": ([C.c_void_p, C.POINTER(C.c_void_p)], C.c_int),
     "lt_graph_launch": ([C.c_void_p, C.c_void_p], C.c_int),
@@ -309,6 +334,16 @@ class DeviceArray:
             sync(stream)
         return out
 
+    def download_range(self, offset, n, stream=0):
+        """Elements [offset, offset + n) to a new host array."""
+        assert 0 <= offset and offset + n <= self.n
+        out = np.empty(int(n), dtype=self.dtype)
+        check(lib().lt_memcpy_d2h(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr + offset * out.itemsize),
+                                  out.nbytes, C.c_void_p(stream)))
+        if stream:
+            sync(stream)
+        return out
+
     def fill_bytes(self, value: int, stream=0):
         check(lib().lt_memset_d(C.c_void_p(self.ptr), int(value), self.nbytes, C.c_void_p(stream)))
 
@@ -342,6 +377,61 @@ class Graph:
     def close(self):
         if self.h:
             lib().lt_graph_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Map-serving loop (lt_server_*): resident models, requests admitted by HBM footprint."""
+
+    PLANES = {MAP_HESSIAN: 4, MAP_FIELDS: 5, MAP_GRADIENT: 2, MAP_DPL: 2}
+
+    def __init__(self, nslots=4, hbm_budget_bytes=32 << 30):
+        self.h = C.c_void_p()
+        check(lib().lt_server_create(C.byref(self.h), nslots, hbm_budget_bytes))
+        self._lenses = []
+
+    def add_model(self, lens: M.LensModel) -> int:
+        mid = C.c_int()
+        check(lib().lt_server_add_model(self.h, C.byref(lens.soa()), lens.n, C.byref(mid)))
+        self._lenses.append(lens)
+        return mid.value
+
+    def submit(self, model, kind, mode, grid: M.Grid, precision=PREC_FP64, r=1.0, mult=1.0) -> int:
+        g = LtGridDesc(grid.frame.xmin, grid.frame.ymin, grid.dx, grid.dy, grid.nx, grid.ny, 0, 0, grid.nx)
+        req = LtRequest(model, kind, mode, precision, g, r, mult)
+        t = C.c_longlong()
+        check(lib().lt_server_submit(self.h, C.byref(req), C.byref(t)))
+        return t.value
+
+    def wait(self, ticket, timeout_ms=-1.0) -> int:
+        return lib().lt_server_wait(self.h, ticket, timeout_ms)
+
+    def fetch(self, ticket, kind, npix, into=None):
+        """Copy the result planes to host arrays (or into the given ones); returns (planes, times)."""
+        n = self.PLANES.get(kind, 1)
+        outs = into if into is not None else [np.empty(npix) for _ in range(n)]
+        ptrs = [o.ctypes.data_as(C.c_void_p) if o is not None else None for o in outs] + [None] * (5 - len(outs))
+        tm = LtRequestTimes()
+        check(lib().lt_server_fetch(self.h, ticket, *ptrs, C.byref(tm)))
+        return outs, tm
+
+    def stats(self) -> dict:
+        st = LtServerStats()
+        check(lib().lt_server_get_stats(self.h, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in LtServerStats._fields_}
+
+    def reset_stats(self):
+        check(lib().lt_server_reset_stats(self.h))
+
+    def close(self):
+        if self.h:
+            lib().lt_server_destroy(self.h)
             self.h = C.c_void_p()
 
     def __del__(self):
