@@ -318,6 +318,11 @@ def main():
 
     prec = lthip.PREC_FP64 if a.precision == "fp64" else lthip.PREC_FP32C
     other = lthip.PREC_FP32C if prec == lthip.PREC_FP64 else lthip.PREC_FP64
+    # launch shape (pixels per thread) picked by the library's autotuner for both precisions, untimed
+    tuned = {}
+    for name, pr in (("fp64", lthip.PREC_FP64), ("fp32c", lthip.PREC_FP32C)):
+        best, ms = model.autotune(lthip.MAP_FIELDS, 0, grid, pr, stream)
+        tuned[name] = {"px": best, "ms_px1_px2_px4": [round(x, 3) for x in ms]}
     per, total_ms, wall = run_maps(model, outs, grid, r, prec, a.steps, a.warmup, stream)
     halo_px = lens.n * grid.npix
     value = a.steps * halo_px / (total_ms * 1e-3)
@@ -347,6 +352,7 @@ def main():
             "parallelism": "single GPU",
         },
         "latency_ms": {"p50": percentile(per, 50), "p99": percentile(per, 99), "mean": avg_ms},
+        "autotune": tuned,
         "wall_s": wall,
         "roofline": {
             "bound": "valu",

@@ -149,6 +149,13 @@ int lt_map_device(lt_model model, int kind, int mode, double* out0, double* out1
                   double* out4, double r, double mult, const lt_grid_desc* grid, int precision, int px,
                   void* stream);
 
+/* Time the map with 1, 2 and 4 pixels per thread (one warm-up and two timed launches each, into
+ * scratch planes) and make the fastest the default (px = 0) for maps of this family, precision,
+ * halo-count octave and pixel-count octave in this process. ms[3] (optional) gets the per-launch
+ * times. Results do not depend on px: only speed does. */
+int lt_autotune(lt_model model, int kind, int mode, const lt_grid_desc* grid, int precision, void* stream,
+                int* best_px, float* ms);
+
 /* ---- captured launch sequences (HIP graphs) ------------------------------------------------------
  * Work issued on a caller stream between lt_graph_begin and lt_graph_end (lt_model_upload,
  * lt_map_device, ...) is captured, not run; lt_graph_launch replays it on any stream with one

@@ -2,6 +2,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
+#include <tuple>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -88,11 +90,40 @@ int ensure(void** buf, size_t* have, size_t need)
     return LT_OK;
 }
 
-int default_px(int kind, int precision)
+// Pixels per thread chosen by lt_autotune, per (kernel family, precision, halo-count and pixel-count
+// octave); 2 where nothing was tuned. Every px gives bit-identical results: only speed differs.
+struct TuneKey {
+    int fam, prec, hoct, noct;
+    bool operator<(const TuneKey& o) const
+    {
+        return std::tie(fam, prec, hoct, noct) < std::tie(o.fam, o.prec, o.hoct, o.noct);
+    }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, int> g_tuned;
+
+int octave(long long v)
 {
-    (void)kind;
-    (void)precision;
-    return 2;
+    int k = 0;
+    while (v > 1) {
+        v >>= 1;
+        ++k;
+    }
+    return k;
+}
+
+TuneKey tune_key(int kind, int precision, int nhalos, long long npix)
+{
+    const int fam = (kind == LT_MAP_GRADIENT || kind == LT_MAP_DPL) ? 0 : kind == LT_MAP_POT ? 2 : 1;
+    const int prec = fam == 1 ? precision : LT_PREC_FP64;  // FP32C exists for the Hessian family only
+    return TuneKey{fam, prec, octave(nhalos), octave(npix)};
+}
+
+int default_px(int kind, int precision, int nhalos, long long npix)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(tune_key(kind, precision, nhalos, npix));
+    return it == g_tuned.end() ? 2 : it->second;
 }
 
 struct DevModel {
@@ -104,7 +135,7 @@ struct DevModel {
 
 // Launch the kernel that produces map `kind` from a device model.
 int launch_map(const DevModel& m, int kind, int mode, double* const outs[5], double r, double mult,
-               const GridDesc& g, int precision, int px, hipStream_t s)
+               const GridDesc& g, int precision, int px, hipStream_t s, int nhalos)
 {
     OutDesc o{};
     o.p0 = outs[0];
@@ -116,7 +147,7 @@ int launch_map(const DevModel& m, int kind, int mode, double* const outs[5], dou
     o.mode = mode;
     o.r = r;
     o.mult = mult;
-    if (px <= 0) px = default_px(kind, precision);
+    if (px <= 0) px = default_px(kind, precision, nhalos, (long long)g.nx * g.ny);
     if (px != 1 && px != 2 && px != 4) return fail(LT_ERR_INVALID, "px must be 0, 1, 2 or 4");
     if (precision != LT_PREC_FP64 && precision != LT_PREC_FP32C)
         return fail(LT_ERR_INVALID, "precision must be LT_PREC_FP64 or LT_PREC_FP32C");
@@ -236,7 +267,7 @@ int host_map(int kind, int mode, Use use, double* const host_out[5], const lt_gr
     g.jstart = jstart;
     g.pitch = nx;
     g.nsamples = 1;
-    if ((st = launch_map(m, kind, mode, dev_out, r, mult, g, precision, 0, S.stream))) return st;
+    if ((st = launch_map(m, kind, mode, dev_out, r, mult, g, precision, 0, S.stream, nhalos))) return st;
     for (int k = 0; k < np; ++k)
         if (host_out[k])
             HIP_TRY(hipMemcpyAsync(host_out[k], dev_out[k], n * sizeof(double), hipMemcpyDeviceToHost, S.stream));
@@ -735,7 +766,7 @@ int lt_map_device(lt_model m, int kind, int mode, double* out0, double* out1, do
     g.halo_stride = m->nhalos;
     g.plane_stride = grid->pitch * (long long)grid->ny;
     HIP_TRY(hipSetDevice(m->device));
-    return launch_map(m->dm, kind, mode, outs, r, mult, g, precision, px, (hipStream_t)stream);
+    return launch_map(m->dm, kind, mode, outs, r, mult, g, precision, px, (hipStream_t)stream, m->nhalos);
 }
 
 int lt_scale_device(double* x, double s, long long n, void* stream)
@@ -776,6 +807,64 @@ int lt_map_epilogue_device(int kind, int mode, double* out0, double* out1, doubl
     hipError_t e = launch_epilogue64(a, b, c, (int64_t)n, o, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "epilogue launch");
     if (!stream) HIP_TRY(hipStreamSynchronize(nullptr));
+    return LT_OK;
+}
+
+// ---- launch-shape autotuner -------------------------------------------------------------------------
+int lt_autotune(lt_model m, int kind, int mode, const lt_grid_desc* grid, int precision, void* stream, int* best_px,
+                float* ms)
+{
+    if (!m || !grid || !best_px) return fail(LT_ERR_INVALID, "NULL argument");
+    int st = check_kind_mode(kind, mode);
+    if (st) return st;
+    if (grid->nx <= 0 || grid->ny <= 0) return fail(LT_ERR_INVALID, "bad grid");
+    HIP_TRY(hipSetDevice(m->device));
+    hipStream_t s = (hipStream_t)stream;
+    const int np = planes_for_kind(kind);
+    const size_t plane = (size_t)grid->nx * grid->ny * m->nsamples;
+    double* buf = nullptr;
+    if (hipMalloc(&buf, plane * np * sizeof(double)) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(LT_ERR_NOMEM, "autotune scratch planes");
+    }
+    double* outs[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    for (int k = 0; k < np; ++k) outs[k] = buf + k * plane;
+    lt_grid_desc g = *grid;
+    g.pitch = g.nx;
+    const int cand[3] = {1, 2, 4};
+    float best = 1e30f;
+    int bpx = 2;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t he = hipEventCreate(&e0);
+    if (he == hipSuccess) he = hipEventCreate(&e1);
+    for (int i = 0; i < 3 && he == hipSuccess && st == LT_OK; ++i) {
+        const int px = (kind == LT_MAP_POT) ? 1 : cand[i];
+        st = lt_map_device(m, kind, mode, outs[0], outs[1], outs[2], outs[3], outs[4], 0.7, 1.0, &g, precision, px,
+                           stream);  // warm
+        if (st == LT_OK) he = hipEventRecord(e0, s);
+        for (int rep = 0; rep < 2 && st == LT_OK && he == hipSuccess; ++rep)
+            st = lt_map_device(m, kind, mode, outs[0], outs[1], outs[2], outs[3], outs[4], 0.7, 1.0, &g, precision,
+                               px, stream);
+        if (st == LT_OK && he == hipSuccess) he = hipEventRecord(e1, s);
+        if (st == LT_OK && he == hipSuccess) he = hipEventSynchronize(e1);
+        float t = 0.f;
+        if (st == LT_OK && he == hipSuccess) he = hipEventElapsedTime(&t, e0, e1);
+        if (ms) ms[i] = t / 2.f;
+        if (st == LT_OK && he == hipSuccess && t < best) {
+            best = t;
+            bpx = px;
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(buf);
+    if (st) return st;
+    if (he != hipSuccess) return hip_fail(he, "autotune timing");
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned[tune_key(kind, precision, m->nhalos, (long long)grid->nx * grid->ny)] = bpx;
+    }
+    *best_px = bpx;
     return LT_OK;
 }
 

@@ -107,6 +107,8 @@ SIGNATURES = {
     "lt_server_get_stats": ([C.c_void_p, C.POINTER(LtServerStats)], C.c_int),
     "lt_server_reset_stats": ([C.c_void_p], C.c_int),
     "lt_server_destroy": ([C.c_void_p], C.c_int),
+    "lt_autotune": ([C.c_void_p, C.c_int, C.c_int, C.POINTER(LtGridDesc), C.c_int, C.c_void_p,
+                     C.POINTER(C.c_int), C.POINTER(C.c_float)], C.c_int),
     "lt_graph_begin": ([C.c_void_p], C.c_int),
     "lt_graph_end": ([C.c_void_p, C.POINTER(C.c_void_p)], C.c_int),
     "lt_graph_launch": ([C.c_void_p, C.c_void_p], C.c_int),
@@ -526,6 +528,14 @@ class Model:
         ptrs = [C.c_void_p(p) if p else None for p in (list(outs) + [0] * 5)[:5]]
         check(lib().lt_map_device(self.h, kind, mode, *ptrs, r, mult, C.byref(g), precision, px,
                                   C.c_void_p(stream)))
+
+    def autotune(self, kind, mode, grid: M.Grid, precision=PREC_FP64, stream=0):
+        """Pick the fastest pixels-per-thread for this map shape; returns (best_px, ms for px 1, 2, 4)."""
+        g = LtGridDesc(grid.frame.xmin, grid.frame.ymin, grid.dx, grid.dy, grid.nx, grid.ny, 0, 0, grid.nx)
+        best = C.c_int()
+        ms = (C.c_float * 3)()
+        check(lib().lt_autotune(self.h, kind, mode, C.byref(g), precision, C.c_void_p(stream), C.byref(best), ms))
+        return best.value, list(ms)
 
     def close(self):
         if self.h:

@@ -103,3 +103,27 @@ def test_batch_rejects_mismatched_samples(hip):
     other._soa = None
     with pytest.raises(hip.LtError):
         hip.Model([base, other], hip.MAP_HESSIAN)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_autotune_keeps_results(hip, precision):
+    """The autotuner only changes the launch shape: maps before and after tuning are identical."""
+    lens = M.config2_lens(seed=8, ngal=30)
+    grid = M.config2_grid(384)
+    print_case(f"autotune precision={precision}", lens, grid)
+    s = hip.stream_create()
+    try:
+        m = hip.Model(lens, hip.MAP_HESSIAN)
+        outs = [hip.DeviceArray(grid.npix) for _ in range(5)]
+        m.map(hip.MAP_FIELDS, 0, [o.ptr for o in outs], grid, r=0.7, precision=precision, px=2, stream=s)
+        hip.sync(s)
+        before = [o.download() for o in outs]
+        best, ms = m.autotune(hip.MAP_FIELDS, 0, grid, precision, s)
+        assert best in (1, 2, 4) and all(t > 0 for t in ms)
+        m.map(hip.MAP_FIELDS, 0, [o.ptr for o in outs], grid, r=0.7, precision=precision, stream=s)
+        hip.sync(s)
+        for b, o in zip(before, outs):
+            np.testing.assert_array_equal(b, o.download())
+        m.close()
+    finally:
+        hip.stream_destroy(s)
