@@ -132,3 +132,41 @@ void mychi_bruteforce_SOA_GPU_grid_gradient_barycentersource(double* chi, int* e
                                                              const struct Potential_SOA* lens,
                                                              const struct grid_param* frame,
                                                              const int* nimages_strongLensing, galaxy* images);
+
+// ---- the reference's CPU entry points of the same path, served by the GPU ------------------------
+// Same declarations as src/grid_gradient_CPU.hpp:38-39, src/grid_gradient2_CPU.hpp:42-44,
+// src/grid_amplif_CPU.hpp:40, src/gradient.hpp:43, src/gradient2.hpp:29, src/delense_CPU.hpp:20,
+// src/chi_computation.hpp:3-4, src/chi_CPU_barycenter.hpp:13, so that callers of the CPU path
+// (e.g. src/chi_CPU_barycenter.cpp, utils/exec/main.cpp) link this library instead. Reference
+// behaviour kept: gradient_grid_CPU's square overload ignores istart/jstart
+// (src/grid_gradient_CPU.cpp:38-41); amplif_grid_CPU mode 5 is kappa = (a + c) / 2 without dlsds
+// (src/grid_amplif_CPU.cpp:50-83), its other modes (absent on the reference CPU) are the GPU maps'.
+void gradient_grid_CPU(type_t* grid_grad_x, type_t* grid_grad_y, const struct grid_param* frame,
+                       const struct Potential_SOA* lens, int Nlens, int nbgridcells, int istart = 0, int jstart = 0);
+void gradient_grid_CPU(type_t* grid_grad_x, type_t* grid_grad_y, const struct grid_param* frame,
+                       const struct Potential_SOA* lens, int Nlens, type_t dx, type_t dy, int nbgridcells_x,
+                       int nbgridcells_y, int istart = 0, int jstart = 0);
+void gradient2_grid_CPU(struct matrix* grid_grad2, const struct grid_param* frame, const struct Potential_SOA* lens,
+                        int Nlens, int nbgridcells, int istart = 0, int jstart = 0);
+void gradient2_grid_CPU(struct matrix* grid_grad2, const struct grid_param* frame, const struct Potential_SOA* lens,
+                        int Nlens, int nbgridcells_x, int nbgridcells_y, int istart = 0, int jstart = 0);
+void gradient2_grid_CPU(struct matrix* grid_grad2, const struct grid_param* frame, const struct Potential_SOA* lens,
+                        int Nlens, type_t dx, type_t dy, int nbgridcells_x, int nbgridcells_y, int istart = 0,
+                        int jstart = 0);
+void amplif_grid_CPU(type_t* amplif, const struct cosmo_param* cosmo, const struct grid_param* frame,
+                     const struct Potential_SOA* lens, int nhalos, int nbgridcells, int mode_amp, type_t z,
+                     int istart = 0, int jstart = 0);
+struct point module_potentialDerivatives_totalGradient_SOA(const struct point* pImage, const struct Potential_SOA* lens,
+                                                           int nhalos);
+struct matrix module_potentialDerivatives_totalGradient2_SOA(const struct point* pImage,
+                                                             const struct Potential_SOA* lens, int nhalos);
+void delense_barycenter(struct point* image_pos, int* locimagesfound, int* numimg, const runmode_param* runmode,
+                        const struct Potential_SOA* lens, const struct grid_param* frame,
+                        const int* nimages_strongLensing, const struct galaxy* sources, double* grid_gradient_x,
+                        double* grid_gradient_y);
+void chi_computation(double* chi, int* images_found, int* numimagesfound, struct point* imagesposition,
+                     const int* nimages_strongLensing, const galaxy* images, const int nsets);
+void mychi_bruteforce_SOA_CPU_grid_gradient_barycentersource(double* chi, int* error, runmode_param* runmode,
+                                                             const struct Potential_SOA* lens,
+                                                             const struct grid_param* frame,
+                                                             const int* nimages_strongLensing, galaxy* images);

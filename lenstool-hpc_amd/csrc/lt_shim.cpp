@@ -3,6 +3,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <vector>
 
 #include "lenstool_hip/lenstool_gpu.hpp"
 #include "lenstool_hip/lt_capi.h"
@@ -422,4 +423,118 @@ void mychi_bruteforce_SOA_GPU_grid_gradient_barycentersource(double* chi, int* e
     int st = lt_chi_barycentersource(chi, error, runmode->nbgridcells, runmode->nsets, runmode->nhalos, L(lens),
                                      G(frame), nimages, reinterpret_cast<const lt_galaxy*>(images));
     report(st, "mychi_bruteforce_SOA_GPU_grid_gradient_barycentersource");
+}
+
+// ---- CPU entry points of the reference, served by the GPU -------------------------------------------
+// gradient_grid_CPU, src/grid_gradient_CPU.cpp:33-41 (square; istart/jstart are not used there)
+void gradient_grid_CPU(type_t* gx, type_t* gy, const grid_param* frame, const Potential_SOA* lens, int nhalos,
+                       int nbgridcells, int istart, int jstart)
+{
+    (void)istart, (void)jstart;
+    const type_t dx = (frame->xmax - frame->xmin) / (nbgridcells - 1);
+    const type_t dy = (frame->ymax - frame->ymin) / (nbgridcells - 1);
+    report(lt_gradient_grid(gx, gy, G(frame), L(lens), nhalos, dx, dy, nbgridcells, nbgridcells, 0, 0, LT_PREC_FP64),
+           "gradient_grid_CPU");
+}
+
+// src/grid_gradient_CPU.cpp:61-66
+void gradient_grid_CPU(type_t* gx, type_t* gy, const grid_param* frame, const Potential_SOA* lens, int nhalos,
+                       type_t dx, type_t dy, int nx, int ny, int istart, int jstart)
+{
+    report(lt_gradient_grid(gx, gy, G(frame), L(lens), nhalos, dx, dy, nx, ny, istart, jstart, LT_PREC_FP64),
+           "gradient_grid_CPU");
+}
+
+namespace {
+// matrix[] (AoS a, b, c, d) from the device planes
+void hessian_aos(matrix* out, const grid_param* frame, const Potential_SOA* lens, int nhalos, type_t dx, type_t dy,
+                 int nx, int ny, int istart, int jstart, const char* fn)
+{
+    const size_t n = (size_t)nx * ny;
+    std::vector<double> a(n), b(n), c(n), d(n);
+    int st = lt_gradient2_grid(a.data(), b.data(), c.data(), d.data(), G(frame), L(lens), nhalos, dx, dy, nx, ny,
+                               istart, jstart, LT_PREC_FP64);
+    if (st == LT_OK)
+        for (size_t i = 0; i < n; ++i) out[i] = matrix{a[i], b[i], c[i], d[i]};
+    report(st, fn);
+}
+}  // namespace
+
+// gradient2_grid_CPU, src/grid_gradient2_CPU.cpp:30-52
+void gradient2_grid_CPU(matrix* out, const grid_param* frame, const Potential_SOA* lens, int nhalos, int nbgridcells,
+                        int istart, int jstart)
+{
+    const type_t dx = (frame->xmax - frame->xmin) / (nbgridcells - 1);
+    const type_t dy = (frame->ymax - frame->ymin) / (nbgridcells - 1);
+    hessian_aos(out, frame, lens, nhalos, dx, dy, nbgridcells, nbgridcells, istart, jstart, "gradient2_grid_CPU");
+}
+
+void gradient2_grid_CPU(matrix* out, const grid_param* frame, const Potential_SOA* lens, int nhalos, int nx, int ny,
+                        int istart, int jstart)
+{
+    const type_t dx = (frame->xmax - frame->xmin) / (nx - 1);
+    const type_t dy = (frame->ymax - frame->ymin) / (ny - 1);
+    hessian_aos(out, frame, lens, nhalos, dx, dy, nx, ny, istart, jstart, "gradient2_grid_CPU");
+}
+
+void gradient2_grid_CPU(matrix* out, const grid_param* frame, const Potential_SOA* lens, int nhalos, type_t dx,
+                        type_t dy, int nx, int ny, int istart, int jstart)
+{
+    hessian_aos(out, frame, lens, nhalos, dx, dy, nx, ny, istart, jstart, "gradient2_grid_CPU");
+}
+
+// amplif_grid_CPU, src/grid_amplif_CPU.cpp:38-46 (mode 5: kappa without dlsds, :50-83)
+void amplif_grid_CPU(type_t* amplif, const cosmo_param* cosmo, const grid_param* frame, const Potential_SOA* lens,
+                     int nhalos, int nbgridcells, int mode_amp, type_t z, int istart, int jstart)
+{
+    const type_t dx = (frame->xmax - frame->xmin) / (nbgridcells - 1);
+    const type_t dy = (frame->ymax - frame->ymin) / (nbgridcells - 1);
+    report(lt_map_grid(LT_MAP_AMPLI, mode_amp, amplif, nullptr, Cz(cosmo), G(frame), L(lens), nhalos, zl_of(lens), z, dx,
+                       dy, nbgridcells, nbgridcells, istart, jstart, LT_PREC_FP64),
+           "amplif_grid_CPU");
+}
+
+// module_potentialDerivatives_totalGradient_SOA, src/gradient.cpp:899-930 (one point per call: use
+// lt_point_gradients for many)
+point module_potentialDerivatives_totalGradient_SOA(const point* p, const Potential_SOA* lens, int nhalos)
+{
+    point g{0., 0.};
+    report(lt_point_gradients(&g.x, &g.y, &p->x, &p->y, 1, L(lens), nhalos),
+           "module_potentialDerivatives_totalGradient_SOA");
+    return g;
+}
+
+// module_potentialDerivatives_totalGradient2_SOA, src/gradient2.cpp:226-254 (a 1 x 1 grid at the point)
+matrix module_potentialDerivatives_totalGradient2_SOA(const point* p, const Potential_SOA* lens, int nhalos)
+{
+    matrix m{0., 0., 0., 0.};
+    lt_grid_param f{p->x, p->x, p->y, p->y, 0, 0, 0};
+    report(lt_gradient2_grid(&m.a, &m.b, &m.c, &m.d, &f, L(lens), nhalos, 0.0, 0.0, 1, 1, 0, 0, LT_PREC_FP64),
+           "module_potentialDerivatives_totalGradient2_SOA");
+    return m;
+}
+
+// delense_barycenter, src/delense_CPU.cpp:6-138
+void delense_barycenter(point* image_pos, int* locimagesfound, int* numimg, const runmode_param* runmode,
+                        const Potential_SOA* lens, const grid_param* frame, const int* nimages, const galaxy* sources,
+                        double* gx, double* gy)
+{
+    delense_barycenter_GPU(image_pos, locimagesfound, numimg, runmode, lens, frame, nimages, sources, gx, gy);
+}
+
+// chi_computation, src/chi_computation.cpp:10-195
+void chi_computation(double* chi, int* images_found, int* numimagesfound, point* imagesposition, const int* nimages,
+                     const galaxy* images, const int nsets)
+{
+    report(lt_chi_computation(chi, images_found, numimagesfound, reinterpret_cast<const lt_point*>(imagesposition),
+                              nimages, reinterpret_cast<const lt_galaxy*>(images), nsets),
+           "chi_computation");
+}
+
+// mychi_bruteforce_SOA_CPU_grid_gradient_barycentersource, src/chi_CPU_barycenter.cpp:58-287
+void mychi_bruteforce_SOA_CPU_grid_gradient_barycentersource(double* chi, int* error, runmode_param* runmode,
+                                                             const Potential_SOA* lens, const grid_param* frame,
+                                                             const int* nimages, galaxy* images)
+{
+    mychi_bruteforce_SOA_GPU_grid_gradient_barycentersource(chi, error, runmode, lens, frame, nimages, images);
 }

@@ -5,9 +5,13 @@
 //         double x, y, b0, angle, epot, rcore, rcut, z, dlsds [n each]
 // output: doubles, in order: gx, gy (square overload), a, b, c, d (dx/dy overload, istart=1, jstart=2),
 //         ampli mode 1, ampli mode 1 through a foreign map function, shear mode 3, mass mode 2,
-//         potential mode 1, dplx, dply; then the int status of lenstool_gpu_last_status().
+//         potential mode 1, dplx, dply; the CPU-named entry points: gradient2_grid_CPU (AoS, dx/dy
+//         overload, istart=1, jstart=2) a and c, amplif_grid_CPU mode 5, gradient_grid_CPU (square) x;
+//         a plane whose first 16 values are module_potentialDerivatives_totalGradient2_SOA (a,b,c,d)
+//         at the first 4 pixels of the diagonal; then the int status of lenstool_gpu_last_status().
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstdlib>
 #include <vector>
 
@@ -100,6 +104,32 @@ int main(int argc, char** argv)
     st();
     put(p0);
     put(p1);
+    std::vector<matrix> aos(N);
+    gradient2_grid_CPU(aos.data(), &frame, &lens, n, dx, dy, nx, ny, 1, 2);
+    st();
+    for (size_t i = 0; i < N; ++i) {
+        p0[i] = aos[i].a;
+        p1[i] = aos[i].c;
+    }
+    put(p0);
+    put(p1);
+    amplif_grid_CPU(p0.data(), &cosmo, &frame, &lens, n, nx, 5, zs);
+    st();
+    put(p0);
+    gradient_grid_CPU(p0.data(), p1.data(), &frame, &lens, n, nx);
+    st();
+    put(p0);
+    std::fill(p0.begin(), p0.end(), 0.0);
+    for (int k = 0; k < 4; ++k) {
+        point q{xmin + k * dx, ymin + k * dy};
+        matrix m = module_potentialDerivatives_totalGradient2_SOA(&q, &lens, n);
+        st();
+        p0[4 * k + 0] = m.a;
+        p0[4 * k + 1] = m.b;
+        p0[4 * k + 2] = m.c;
+        p0[4 * k + 3] = m.d;
+    }
+    put(p0);
     FILE* o = std::fopen(argv[2], "wb");
     if (!o) return 5;
     std::fwrite(out.data(), 8, out.size(), o);

@@ -40,7 +40,7 @@ def test_reference_signatures(oracle):
     status = struct.unpack("<i", raw[-4:])[0]
     v = np.frombuffer(raw[:-4], dtype="<f8").reshape(-1, n, n)
     assert status == 0
-    assert v.shape[0] == 13
+    assert v.shape[0] == 18
     gx, gy = lthip.gradient_grid(lens, grid)
     np.testing.assert_array_equal(v[0], gx)
     np.testing.assert_array_equal(v[1], gy)
@@ -58,6 +58,15 @@ def test_reference_signatures(oracle):
     dx, dy = lthip.map_grid(lthip.MAP_DPL, 1, lens, grid, cosmo, 0.4, zs)
     np.testing.assert_array_equal(v[11], dx)
     np.testing.assert_array_equal(v[12], dy)
+    # CPU-named entry points served by the GPU: the same numbers as the GPU-named ones
+    np.testing.assert_array_equal(v[13], H[0])
+    np.testing.assert_array_equal(v[14], H[2])
+    H0 = lthip.hessian_grid(lens, grid)
+    np.testing.assert_array_equal(v[15], 0.5 * (H0[0] + H0[2]))  # amplif_grid_CPU mode 5: kappa, no dlsds
+    np.testing.assert_array_equal(v[16], gx)
+    pts = v[17].ravel()[:16].reshape(4, 4)
+    for k in range(4):
+        np.testing.assert_array_equal(pts[k], [H0[0][k, k], H0[1][k, k], H0[2][k, k], H0[3][k, k]])
     # and the shim's gradient is the oracle's within the FP64 criterion
     ex, ey = oracle.gradient_grid(lens, grid, extended=True)
     _, _, ax, ay = oracle.gradient_grid(lens, grid, with_abs=True)
