@@ -42,16 +42,16 @@ def percentile(v, q):
     return float(np.percentile(np.asarray(v), q))
 
 
-def run_maps(model, outs, grid, r, precision, steps, warmup, stream):
+def run_maps(model, outs, grid, r, precision, steps, warmup, stream, px=0):
     ptrs = [o.ptr for o in outs]
     for _ in range(warmup):
-        model.map(lthip.MAP_FIELDS, 0, ptrs, grid, r=r, precision=precision, stream=stream)
+        model.map(lthip.MAP_FIELDS, 0, ptrs, grid, r=r, precision=precision, stream=stream, px=px)
     lthip.sync(stream)
     ev = [lthip.Event() for _ in range(steps + 1)]
     t0 = time.perf_counter()
     ev[0].record(stream)
     for k in range(steps):
-        model.map(lthip.MAP_FIELDS, 0, ptrs, grid, r=r, precision=precision, stream=stream)
+        model.map(lthip.MAP_FIELDS, 0, ptrs, grid, r=r, precision=precision, stream=stream, px=px)
         ev[k + 1].record(stream)
     ev[-1].synchronize()
     wall = time.perf_counter() - t0
@@ -292,6 +292,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision", choices=("fp64", "fp32c"), default="fp64")
     ap.add_argument("--no-secondary", action="store_true", help="do not time the other precision")
+    ap.add_argument("--px", type=int, default=0, help="pixels per thread (0: pick with the autotuner)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="also run one device copy kernel of this many bytes (hardware-counter calibration)")
     a = ap.parse_args()
@@ -321,9 +322,12 @@ def main():
     # launch shape (pixels per thread) picked by the library's autotuner for both precisions, untimed
     tuned = {}
     for name, pr in (("fp64", lthip.PREC_FP64), ("fp32c", lthip.PREC_FP32C)):
+        if a.px:
+            tuned[name] = {"px": a.px, "ms_px1_px2_px4": None}
+            continue
         best, ms = model.autotune(lthip.MAP_FIELDS, 0, grid, pr, stream)
         tuned[name] = {"px": best, "ms_px1_px2_px4": [round(x, 3) for x in ms]}
-    per, total_ms, wall = run_maps(model, outs, grid, r, prec, a.steps, a.warmup, stream)
+    per, total_ms, wall = run_maps(model, outs, grid, r, prec, a.steps, a.warmup, stream, a.px)
     halo_px = lens.n * grid.npix
     value = a.steps * halo_px / (total_ms * 1e-3)
     avg_ms = float(np.mean(per))
@@ -367,7 +371,7 @@ def main():
     if not a.no_secondary:
         # the other precision on the same steps, then the FP32C error against FP64 (untimed)
         first = [o.download() for o in outs[:4]]
-        per2, total2, _ = run_maps(model, outs, grid, r, other, a.steps, a.warmup, stream)
+        per2, total2, _ = run_maps(model, outs, grid, r, other, a.steps, a.warmup, stream, a.px)
         second = [o.download() for o in outs[:4]]
         f64, f32 = (first, second) if prec == lthip.PREC_FP64 else (second, first)
         errs = {n: float(np.abs(x - y).max()) for n, x, y in zip(("kappa", "gamma1", "gamma2", "detA"), f32, f64)}

@@ -57,8 +57,9 @@ struct alignas(64) HaloRec32 {
     float a2, b2;                       // 6, 7    a^2 = 1/(1+e)^2, b^2 = 1/(1-e)^2
     float e2, cx1, e2ab, e2a2;          // 8..11   2 sqrt(e), (1-e)/(1+e), 2 sqrt(e) a b, 2 sqrt(e) a^2
     float cxa2;                         // 12      cx1 a^2
-    float ra2, e2ra, rcx1a;             // 13..15  rcore^2, 2 rcore sqrt(e), rcore cx1
-    float rb2, e2rb, rcx1b;             // 16..18  the same for rcut
+    float ra, ra2;                      // 13, 14  rcore, rcore^2
+    float rb, rb2;                      // 15, 16  rcut, rcut^2
+    float icx1, pad0;                   // 17, 18  1 / cx1
     float hT, hc2, hs2m2, hs2h;         // 19..22  lab-frame amplitudes: trace, cos 2t, -2 sin 2t, sin 2t / 2
     float near2;                        // 23
     float sa, sb;                       // 24, 25  SIS: 1 - e, 1 + e

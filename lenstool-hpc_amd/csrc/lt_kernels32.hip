@@ -14,14 +14,16 @@
 // pieces N'/N and D'/D are both singular at D = 0, i.e. at halo-frame (0, 2 r s); the closed form
 // cancels them there and in FP32 loses ~1e-5 relative over the whole upper half plane. Writing
 // v = r cx1 + 2 s a b y, one has w^2 - v^2 = a^2 |D|^2 and N = cx1 D + 2 i s (w - v), hence
-//   N / D = Q / S,  S = w + v,  Q = cx1 S + 2 s a^2 (n2 + i x),  n2 = 2 r s - y,
-// and with U = b^2 y / w + 2 s a b the three second derivatives are, up to the factor 2 s a^2 ci:
+//   N / D = Q / S,  S = w + v,  Q = r + cx1 w + 2 i s a^2 x
+// (Re Q = cx1 S + 2 s a^2 (2 r s - y) = r + cx1 w because cx1 b = a and a (1 + e) = 1). With
+// U = b^2 y / w + 2 s a b and t = 2 s w - y / cx1 the three second derivatives are, up to the
+// factor 2 s a^2 ci:
 //   phi_xx ~ (Re Q - cx1 a^2 x^2 / w) / |Q|^2
 //   phi_xy ~ x (2 s a^2 - cx1 U) / |Q|^2
-//   phi_yy ~ (Re Q + U (cx1 n2 + 2 s (w - v))) / |Q|^2
-// Q never vanishes and S > 0, so the form has no cancellation near D = 0; it needs one rsq and one
-// rcp per core radius. (Derivation: differentiate ln Q - ln S with dw/dx = a^2 x / w,
-// dw/dy = b^2 y / w, dv/dy = 2 s a b; |D|^2 / S = (w - v) / a^2.)
+//   phi_yy ~ (Re Q + U t) / |Q|^2
+// Re Q >= r > 0, so |Q| never vanishes and nothing cancels near D = 0; one rsq and one rcp per core
+// radius. (Derivation: differentiate ln Q - ln S with dw/dx = a^2 x / w, dw/dy = b^2 y / w,
+// dv/dy = 2 s a b, and use |D|^2 / S = (w - v) / a^2, 2 s (w - v) + cx1 (2 r s - y) = t.)
 #include <hip/hip_runtime.h>
 
 #include "lenstool_hip/lt_status.h"
@@ -52,13 +54,14 @@ __device__ __forceinline__ Frame32 frame32(const HaloRec32& h, float pxh, float 
     return f;
 }
 
+// numerators of (xx + yy), (xx - yy), xy and the common factor 1 / |Q|^2
 struct Q32 {
-    float xx, xy, yy;
+    float Tn, Dn, Bn, iq;
 };
 
 // pixel-only quantities shared by the core and cut terms
 struct QShared {
-    float wq0, e2aby, Qi2, cxa2x2, b2y, e2a2x, cx1x;
+    float wq0, Qi2, cxa2x2, b2y, e2a2x, cx1x, yicx1;
 };
 
 __device__ __forceinline__ QShared q_shared(const HaloRec32& h, float x, float y)
@@ -66,55 +69,55 @@ __device__ __forceinline__ QShared q_shared(const HaloRec32& h, float x, float y
     QShared s;
     const float x2 = x * x;
     s.wq0 = __builtin_fmaf(h.a2, x2, h.b2 * (y * y));
-    s.e2aby = h.e2ab * y;
     s.e2a2x = h.e2a2 * x;
     s.Qi2 = s.e2a2x * s.e2a2x;
     s.cxa2x2 = h.cxa2 * x2;
     s.b2y = h.b2 * y;
     s.cx1x = h.cx1 * x;
+    s.yicx1 = y * h.icx1;
     return s;
 }
 
-__device__ __forceinline__ Q32 q_term(const HaloRec32& h, const QShared& s, float y, float r2, float e2r,
-                                      float rcx1)
+__device__ __forceinline__ Q32 q_term(const HaloRec32& h, const QShared& s, float r, float r2)
 {
     const float w2 = s.wq0 + r2;
     const float iw = rsq32(w2);
     const float w = w2 * iw;
-    const float v = s.e2aby + rcx1;
-    const float n2 = e2r - y;
-    const float Qr = __builtin_fmaf(h.e2a2, n2, h.cx1 * (w + v));
-    const float iq = rcp32(__builtin_fmaf(Qr, Qr, s.Qi2));
+    const float Qr = __builtin_fmaf(h.cx1, w, r);
     const float U = __builtin_fmaf(s.b2y, iw, h.e2ab);
+    const float xx = __builtin_fmaf(-s.cxa2x2, iw, Qr);
+    const float xy = __builtin_fmaf(-s.cx1x, U, s.e2a2x);
+    const float yy = __builtin_fmaf(U, __builtin_fmaf(h.e2, w, -s.yicx1), Qr);
     Q32 o;
-    o.xx = __builtin_fmaf(-s.cxa2x2, iw, Qr) * iq;
-    o.xy = __builtin_fmaf(-s.cx1x, U, s.e2a2x) * iq;
-    o.yy = __builtin_fmaf(U, __builtin_fmaf(h.e2, w - v, h.cx1 * n2), Qr) * iq;
+    o.Tn = xx + yy;
+    o.Dn = xx - yy;
+    o.Bn = xy;
+    o.iq = rcp32(__builtin_fmaf(Qr, Qr, s.Qi2));
     return o;
 }
 
-// add one halo-frame Hessian (xx, xy, yy), already differenced over the core radii, in the lab frame
-__device__ __forceinline__ void acc_lab(const HaloRec32& h, float xx, float xy, float yy, float& fT, float& fD,
+// add one halo-frame Hessian (trace T = xx + yy, difference D = xx - yy, off-diagonal B = xy), already
+// differenced over the core radii, in the lab frame
+__device__ __forceinline__ void acc_lab(const HaloRec32& h, float T, float D, float B, float& fT, float& fD,
                                         float& fB)
 {
-    const float T = xx + yy, D = xx - yy;
     fT = __builtin_fmaf(h.hT, T, fT);
-    fD = __builtin_fmaf(h.hc2, D, __builtin_fmaf(h.hs2m2, xy, fD));
-    fB = __builtin_fmaf(h.hs2h, D, __builtin_fmaf(h.hc2, xy, fB));
+    fD = __builtin_fmaf(h.hc2, D, __builtin_fmaf(h.hs2m2, B, fD));
+    fB = __builtin_fmaf(h.hs2h, D, __builtin_fmaf(h.hc2, B, fB));
 }
 
 template <bool CUT>
 __device__ __forceinline__ void hess_piemd32(const HaloRec32& h, const Frame32& f, float& fT, float& fD, float& fB)
 {
     const QShared s = q_shared(h, f.x, f.y);
-    Q32 a = q_term(h, s, f.y, h.ra2, h.e2ra, h.rcx1a);
+    const Q32 a = q_term(h, s, h.ra, h.ra2);
     if (CUT) {
-        const Q32 b = q_term(h, s, f.y, h.rb2, h.e2rb, h.rcx1b);
-        a.xx -= b.xx;
-        a.xy -= b.xy;
-        a.yy -= b.yy;
+        const Q32 b = q_term(h, s, h.rb, h.rb2);
+        acc_lab(h, __builtin_fmaf(a.Tn, a.iq, -(b.Tn * b.iq)), __builtin_fmaf(a.Dn, a.iq, -(b.Dn * b.iq)),
+                __builtin_fmaf(a.Bn, a.iq, -(b.Bn * b.iq)), fT, fD, fB);
+    } else {
+        acc_lab(h, a.Tn * a.iq, a.Dn * a.iq, a.Bn * a.iq, fT, fD, fB);
     }
-    acc_lab(h, a.xx, a.xy, a.yy, fT, fD, fB);
 }
 
 // SIS / SIE: halo-frame Hessian amph / R^3 (y^2, -xy, x^2), R^2 = (1-e) x^2 + (1+e) y^2
@@ -123,7 +126,7 @@ __device__ __forceinline__ void hess_sis32(const HaloRec32& h, const Frame32& f,
     const float x2 = f.x * f.x, y2 = f.y * f.y;
     const float iR = rsq32(__builtin_fmaf(h.sa, x2, h.sb * y2));
     const float g = iR * iR * iR;
-    acc_lab(h, g * y2, -(g * f.x * f.y), g * x2, fT, fD, fB);
+    acc_lab(h, g * (x2 + y2), g * (y2 - x2), -(g * f.x * f.y), fT, fD, fB);
 }
 
 // One run of halos of one type: FP32 terms summed per chunk of CHUNK halos, FP64 terms (near

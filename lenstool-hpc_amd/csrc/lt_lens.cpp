@@ -141,12 +141,11 @@ int pack_model(const lt_potential_soa* L, int n, Use use, std::vector<HaloRec>& 
             f.e2ab = (float)(e2 * a * b);
             f.e2a2 = (float)(e2 * a * a);
             f.cxa2 = (float)(h.cx1 * a * a);
+            f.ra = (float)h.rc;
             f.ra2 = (float)h.ra2;
-            f.e2ra = (float)h.e2ra;
-            f.rcx1a = (float)(h.rc * h.cx1);
+            f.rb = (float)h.rcut;
             f.rb2 = (float)h.rb2;
-            f.e2rb = (float)h.e2rb;
-            f.rcx1b = (float)(h.rcut * h.cx1);
+            f.icx1 = (float)h.icx1;
             f.hT = (float)amp;
             f.hc2 = (float)(amp * cos2);
             f.hs2m2 = (float)(-2.0 * amp * sin2);
