@@ -14,7 +14,8 @@ namespace lt {
 // One halo, 40 doubles. Field meaning by type:
 //   all     : xh, yh (centre), cs, sn (cos/sin of the angle), hc2, hs2 (Hessian amplitude times
 //             cos/sin of twice the angle), b0, eps, rc, rcut (raw parameters, for the potential)
-//   8, 81   : PIEMD constants (see below); ampg / amph gradient / Hessian amplitudes
+//   8, 81   : PIEMD constants (see below); ampg / amph gradient / Hessian amplitudes; e2ab .. qs2h
+//             the constants of the division-free Hessian form
 //   5       : sa = 1 - eps, sb = 1 + eps, ampg = b0, amph = b0 (1 - eps)(1 + eps)
 //   14      : h14a, h14b, h14c = constant Hessian (lab frame)
 struct alignas(64) HaloRec {
@@ -31,7 +32,8 @@ struct alignas(64) HaloRec {
     double sa, sb;                  // 24, 25 SIS (1 - e), (1 + e)
     double h14a, h14b, h14c;        // 26..28 type-14 constant Hessian
     double b0, eps, rc, rcut;       // 29..32 raw parameters (potential)
-    double pad[7];                  // 33..39
+    double e2ab, e2a2, cxa2;        // 33..35 2 sqrt(e) a b, 2 sqrt(e) a^2, cx1 a^2 (a, b = 1/(1 +- e))
+    double qT, qc2, qs2m2, qs2h;    // 36..39 Hessian amplitudes of the division-free form (lt_terms64.h)
 };
 static_assert(sizeof(HaloRec) == 320, "halo record is 320 bytes");
 

@@ -120,6 +120,16 @@ int pack_model(const lt_potential_soa* L, int n, Use use, std::vector<HaloRec>& 
         }
         h.hc2 = h.amph * cos2;
         h.hs2 = h.amph * sin2;
+        if (piemd) {
+            const double a = 1.0 / (1.0 + eps), b = 1.0 / (1.0 - eps);
+            h.e2ab = h.e2 * a * b;
+            h.e2a2 = h.e2 * a * a;
+            h.cxa2 = h.cx1 * a * a;
+            h.qT = h.amph * h.e2a2;
+            h.qc2 = h.hc2 * h.e2a2;
+            h.qs2m2 = -2.0 * h.hs2 * h.e2a2;
+            h.qs2h = 0.5 * h.hs2 * h.e2a2;
+        }
         recs[i] = h;
 
         HaloRec32& f = recs32[i];

@@ -39,33 +39,52 @@ __device__ __forceinline__ PixelFrame halo_frame(const HaloRec& h, double px, do
     return f;
 }
 
-// One I_{0.5} term of the PIEMD Hessian in the halo frame (Kassiola & Kovner 1993), split into
-// trace / difference / off-diagonal parts. The den2 pieces cancel in the trace.
-struct PiemdH {
-    double T, D, B;
+// One I_{0.5} term of the PIEMD Hessian in the halo frame (Kassiola & Kovner 1993) in the
+// division-free form: with s = sqrt(e), a = 1/(1+e), b = 1/(1-e), cx1 = a/b and
+// w = sqrt(r^2 + a^2 x^2 + b^2 y^2), the reference's N / D (src/gradient2.cpp:54-82) equals Q / S with
+// Q = r + cx1 w + 2 i s a^2 x and S = w + r cx1 + 2 s a b y, and with U = b^2 y / w + 2 s a b,
+// t = 2 s w - y / cx1 the second derivatives are, up to the factor 2 s a^2 ci:
+//   phi_xx ~ (Re Q - cx1 a^2 x^2 / w) / |Q|^2,  phi_xy ~ x (2 s a^2 - cx1 U) / |Q|^2,
+//   phi_yy ~ (Re Q + U t) / |Q|^2.
+// Re Q >= r > 0: unlike the closed form, whose two pieces are both singular at halo-frame
+// (0, 2 r s) and cancel there, nothing cancels anywhere. Returned: numerators of the trace
+// (xx + yy), difference (xx - yy) and xy, and 1 / |Q|^2.
+struct PiemdQ {
+    double Tn, Dn, Bn, iq;
 };
 
-__device__ __forceinline__ PiemdH piemd_hess_term(const HaloRec& h, const PixelFrame& f, double rem2,
-                                                  double yi, double cxx, double X1, double Y6,
-                                                  double Y5, double XY3, double r2, double e2r)
+struct PiemdShared {
+    double wq0, Qi2, cxa2x2, b2y, e2a2x, cx1x, yicx1;
+};
+
+__device__ __forceinline__ PiemdShared piemd_shared(const HaloRec& h, const PixelFrame& f)
 {
-    double q = r2 + rem2;
-    double iw = rsq64(q);
-    double w = q * iw;
-    double t = __builtin_fma(h.e2, w, -yi);
-    double den1 = __builtin_fma(t, t, cxx);
-    double n2 = e2r - f.y;
-    double den2 = __builtin_fma(n2, n2, f.x2);
-    double ip = rcp64(den1 * den2);
-    double id1 = den2 * ip;
-    double id2 = den1 * ip;
-    double bxx = __builtin_fma(X1, iw, __builtin_fma(-h.hk2, w, f.y));
-    double byy = __builtin_fma(h.hk4, w, __builtin_fma(Y6, iw, -Y5));
-    double bxy = __builtin_fma(XY3, iw, -f.x);
-    PiemdH o;
-    o.T = (bxx + byy) * id1;
-    o.D = __builtin_fma(bxx - byy, id1, (n2 + n2) * id2);
-    o.B = __builtin_fma(bxy, id1, f.x * id2);
+    PiemdShared s;
+    s.wq0 = __builtin_fma(h.kx, f.x2, h.ky * f.y2);
+    s.e2a2x = h.e2a2 * f.x;
+    s.Qi2 = s.e2a2x * s.e2a2x;
+    s.cxa2x2 = h.cxa2 * f.x2;
+    s.b2y = h.ky * f.y;
+    s.cx1x = h.cx1 * f.x;
+    s.yicx1 = f.y * h.icx1;
+    return s;
+}
+
+__device__ __forceinline__ PiemdQ piemd_q_term(const HaloRec& h, const PiemdShared& s, double r, double r2)
+{
+    const double w2 = s.wq0 + r2;
+    const double iw = rsq64(w2);
+    const double w = w2 * iw;
+    const double Qr = __builtin_fma(h.cx1, w, r);
+    const double U = __builtin_fma(s.b2y, iw, h.e2ab);
+    const double xx = __builtin_fma(-s.cxa2x2, iw, Qr);
+    const double xy = __builtin_fma(-s.cx1x, U, s.e2a2x);
+    const double yy = __builtin_fma(U, __builtin_fma(h.e2, w, -s.yicx1), Qr);
+    PiemdQ o;
+    o.Tn = xx + yy;
+    o.Dn = xx - yy;
+    o.Bn = xy;
+    o.iq = rcp64(__builtin_fma(Qr, Qr, s.Qi2));
     return o;
 }
 
@@ -73,27 +92,23 @@ template <bool CUT>
 __device__ __forceinline__ void hess_piemd(const HaloRec& h, double px, double py, double& aT, double& aD,
                                            double& aB)
 {
-    PixelFrame f = halo_frame(h, px, py);
-    double rem2 = __builtin_fma(f.x2, h.kx, f.y2 * h.ky);
-    double yi = f.y * h.icx1;
-    double cxx = h.cx1sq * f.x2;
-    double X1 = h.hk1 * f.x2;
-    double Y6 = h.hk6 * f.y2;
-    double Y5 = h.hk5 * f.y;
-    double XY3 = h.hk3 * f.x * f.y;
-    PiemdH a = piemd_hess_term(h, f, rem2, yi, cxx, X1, Y6, Y5, XY3, h.ra2, h.e2ra);
-    double dT = a.T, dD = a.D, dB = a.B;
+    const PixelFrame f = halo_frame(h, px, py);
+    const PiemdShared s = piemd_shared(h, f);
+    const PiemdQ a = piemd_q_term(h, s, h.rc, h.ra2);
+    double dT, dD, dB;
     if (CUT) {
-        PiemdH b = piemd_hess_term(h, f, rem2, yi, cxx, X1, Y6, Y5, XY3, h.rb2, h.e2rb);
-        dT -= b.T;
-        dD -= b.D;
-        dB -= b.B;
+        const PiemdQ b = piemd_q_term(h, s, h.rcut, h.rb2);
+        dT = __builtin_fma(a.Tn, a.iq, -(b.Tn * b.iq));
+        dD = __builtin_fma(a.Dn, a.iq, -(b.Dn * b.iq));
+        dB = __builtin_fma(a.Bn, a.iq, -(b.Bn * b.iq));
+    } else {
+        dT = a.Tn * a.iq;
+        dD = a.Dn * a.iq;
+        dB = a.Bn * a.iq;
     }
-    aT = __builtin_fma(h.amph, dT, aT);
-    aD = __builtin_fma(h.hc2, dD, aD);
-    aD = __builtin_fma(-2.0 * h.hs2, dB, aD);
-    aB = __builtin_fma(0.5 * h.hs2, dD, aB);
-    aB = __builtin_fma(h.hc2, dB, aB);
+    aT = __builtin_fma(h.qT, dT, aT);
+    aD = __builtin_fma(h.qc2, dD, __builtin_fma(h.qs2m2, dB, aD));
+    aB = __builtin_fma(h.qs2h, dD, __builtin_fma(h.qc2, dB, aB));
 }
 
 // SIS / SIE: phi = b0 R, R^2 = (1-e) x^2 + (1+e) y^2; halo-frame Hessian
