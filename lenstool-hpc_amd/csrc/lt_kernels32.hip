@@ -44,19 +44,29 @@ struct Frame32 {
     float x, y;
 };
 
-__device__ __forceinline__ Frame32 frame32(const HaloRec32& h, float pxh, float pxl, float pyh, float pyl)
+// the row part (the same for every pixel of a thread): ty sin, ty cos
+struct Row32 {
+    float tys, tyc;
+};
+
+__device__ __forceinline__ Row32 row32(const HaloRec32& h, float pyh, float pyl)
+{
+    const float ty = (pyh - h.yh_hi) + (pyl - h.yh_lo);
+    return Row32{ty * h.sn, ty * h.cs};
+}
+
+__device__ __forceinline__ Frame32 frame32(const HaloRec32& h, const Row32& r, float pxh, float pxl)
 {
     const float tx = (pxh - h.xh_hi) + (pxl - h.xh_lo);
-    const float ty = (pyh - h.yh_hi) + (pyl - h.yh_lo);
     Frame32 f;
-    f.x = __builtin_fmaf(tx, h.cs, ty * h.sn);
-    f.y = __builtin_fmaf(ty, h.cs, -(tx * h.sn));
+    f.x = __builtin_fmaf(tx, h.cs, r.tys);
+    f.y = __builtin_fmaf(-tx, h.sn, r.tyc);
     return f;
 }
 
-// numerators of (xx + yy), (xx - yy), xy and the common factor 1 / |Q|^2
+// numerators of xx, xy, yy and their common factor 1 / |Q|^2
 struct Q32 {
-    float Tn, Dn, Bn, iq;
+    float xx, xy, yy, iq;
 };
 
 // pixel-only quantities shared by the core and cut terms
@@ -85,13 +95,10 @@ __device__ __forceinline__ Q32 q_term(const HaloRec32& h, const QShared& s, floa
     const float w = w2 * iw;
     const float Qr = __builtin_fmaf(h.cx1, w, r);
     const float U = __builtin_fmaf(s.b2y, iw, h.e2ab);
-    const float xx = __builtin_fmaf(-s.cxa2x2, iw, Qr);
-    const float xy = __builtin_fmaf(-s.cx1x, U, s.e2a2x);
-    const float yy = __builtin_fmaf(U, __builtin_fmaf(h.e2, w, -s.yicx1), Qr);
     Q32 o;
-    o.Tn = xx + yy;
-    o.Dn = xx - yy;
-    o.Bn = xy;
+    o.xx = __builtin_fmaf(-s.cxa2x2, iw, Qr);
+    o.xy = __builtin_fmaf(-s.cx1x, U, s.e2a2x);
+    o.yy = __builtin_fmaf(U, __builtin_fmaf(h.e2, w, -s.yicx1), Qr);
     o.iq = rcp32(__builtin_fmaf(Qr, Qr, s.Qi2));
     return o;
 }
@@ -111,13 +118,18 @@ __device__ __forceinline__ void hess_piemd32(const HaloRec32& h, const Frame32& 
 {
     const QShared s = q_shared(h, f.x, f.y);
     const Q32 a = q_term(h, s, h.ra, h.ra2);
+    float xx, xy, yy;
     if (CUT) {
         const Q32 b = q_term(h, s, h.rb, h.rb2);
-        acc_lab(h, __builtin_fmaf(a.Tn, a.iq, -(b.Tn * b.iq)), __builtin_fmaf(a.Dn, a.iq, -(b.Dn * b.iq)),
-                __builtin_fmaf(a.Bn, a.iq, -(b.Bn * b.iq)), fT, fD, fB);
+        xx = __builtin_fmaf(a.xx, a.iq, -(b.xx * b.iq));
+        xy = __builtin_fmaf(a.xy, a.iq, -(b.xy * b.iq));
+        yy = __builtin_fmaf(a.yy, a.iq, -(b.yy * b.iq));
     } else {
-        acc_lab(h, a.Tn * a.iq, a.Dn * a.iq, a.Bn * a.iq, fT, fD, fB);
+        xx = a.xx * a.iq;
+        xy = a.xy * a.iq;
+        yy = a.yy * a.iq;
     }
+    acc_lab(h, xx + yy, xx - yy, xy, fT, fD, fB);
 }
 
 // SIS / SIE: halo-frame Hessian amph / R^3 (y^2, -xy, x^2), R^2 = (1-e) x^2 + (1+e) y^2
@@ -143,9 +155,10 @@ __device__ __forceinline__ void run32(const HaloRec* __restrict__ h64, const Hal
         for (int k = 0; k < PX; ++k) fT[k] = fD[k] = fB[k] = 0.f;
         for (int i = c0; i < c1; ++i) {
             const HaloRec32& h = h32[i];
+            const Row32 rw = row32(h, pyh, pyl);
 #pragma unroll
             for (int k = 0; k < PX; ++k) {
-                const Frame32 f = frame32(h, pxh[k], pxl[k], pyh, pyl);
+                const Frame32 f = frame32(h, rw, pxh[k], pxl[k]);
                 if (__builtin_fmaf(f.x, f.x, f.y * f.y) < h.near2) {
                     const HaloRec& hd = h64[i];
                     if constexpr (TYPE == TYPE_DPIE) hess_piemd<true>(hd, px[k], py, sT[k], sD[k], sB[k]);

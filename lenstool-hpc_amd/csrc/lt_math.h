@@ -5,29 +5,26 @@
 
 namespace lt {
 
-// 1/x in FP64: hardware reciprocal estimate refined by two Newton steps (each squares the
-// relative error), for the strictly positive, normal-range denominators of the closed forms.
+// 1/x in FP64 for the strictly positive, normal-range denominators of the closed forms: hardware
+// reciprocal estimate r (relative error d below 2^-24 on gfx950), then one third-order correction
+// 1/x = r / (1 - e) ~ r (1 + e + e^2) with e = 1 - x r = d, whose truncation error e^3 < 2^-72 is far
+// below the rounding of the result (two FMAs and one multiply-free FMA).
 __device__ __forceinline__ double rcp64(double x)
 {
     double r = __builtin_amdgcn_rcp(x);
     double e = __builtin_fma(-x, r, 1.0);
-    r = __builtin_fma(r, e, r);
-    e = __builtin_fma(-x, r, 1.0);
-    r = __builtin_fma(r, e, r);
-    return r;
+    return __builtin_fma(r, __builtin_fma(e, e, e), r);
 }
 
-// 1/sqrt(q) in FP64 for q > 0: hardware estimate plus two Newton steps
-// y <- y + y (1 - q y^2)/2.
+// 1/sqrt(q) in FP64 for q > 0: hardware estimate y (relative error below 2^-24), then one
+// third-order correction q^(-1/2) = y (1 - e)^(-1/2) ~ y (1 + e/2 + 3 e^2/8) with e = 1 - q y^2;
+// truncation error about 5 e^3 / 16 < 2^-70.
 __device__ __forceinline__ double rsq64(double q)
 {
     double y = __builtin_amdgcn_rsq(q);
-    double hq = 0.5 * q;
-    double e = __builtin_fma(-hq * y, y, 0.5);
-    y = __builtin_fma(y, e, y);
-    e = __builtin_fma(-hq * y, y, 0.5);
-    y = __builtin_fma(y, e, y);
-    return y;
+    double e = __builtin_fma(-(q * y), y, 1.0);
+    double p = __builtin_fma(0.375, e, 0.5);
+    return __builtin_fma(y, e * p, y);
 }
 
 // Pixel coordinate exactly as the reference CPU forms it: xmin + (i * d), two roundings, no fused

@@ -47,10 +47,10 @@ __device__ __forceinline__ PixelFrame halo_frame(const HaloRec& h, double px, do
 //   phi_xx ~ (Re Q - cx1 a^2 x^2 / w) / |Q|^2,  phi_xy ~ x (2 s a^2 - cx1 U) / |Q|^2,
 //   phi_yy ~ (Re Q + U t) / |Q|^2.
 // Re Q >= r > 0: unlike the closed form, whose two pieces are both singular at halo-frame
-// (0, 2 r s) and cancel there, nothing cancels anywhere. Returned: numerators of the trace
-// (xx + yy), difference (xx - yy) and xy, and 1 / |Q|^2.
+// (0, 2 r s) and cancel there, nothing cancels anywhere. Returned: the numerators of xx, xy, yy
+// and 1 / |Q|^2.
 struct PiemdQ {
-    double Tn, Dn, Bn, iq;
+    double xx, xy, yy, iq;
 };
 
 struct PiemdShared {
@@ -77,13 +77,10 @@ __device__ __forceinline__ PiemdQ piemd_q_term(const HaloRec& h, const PiemdShar
     const double w = w2 * iw;
     const double Qr = __builtin_fma(h.cx1, w, r);
     const double U = __builtin_fma(s.b2y, iw, h.e2ab);
-    const double xx = __builtin_fma(-s.cxa2x2, iw, Qr);
-    const double xy = __builtin_fma(-s.cx1x, U, s.e2a2x);
-    const double yy = __builtin_fma(U, __builtin_fma(h.e2, w, -s.yicx1), Qr);
     PiemdQ o;
-    o.Tn = xx + yy;
-    o.Dn = xx - yy;
-    o.Bn = xy;
+    o.xx = __builtin_fma(-s.cxa2x2, iw, Qr);
+    o.xy = __builtin_fma(-s.cx1x, U, s.e2a2x);
+    o.yy = __builtin_fma(U, __builtin_fma(h.e2, w, -s.yicx1), Qr);
     o.iq = rcp64(__builtin_fma(Qr, Qr, s.Qi2));
     return o;
 }
@@ -95,17 +92,18 @@ __device__ __forceinline__ void hess_piemd(const HaloRec& h, double px, double p
     const PixelFrame f = halo_frame(h, px, py);
     const PiemdShared s = piemd_shared(h, f);
     const PiemdQ a = piemd_q_term(h, s, h.rc, h.ra2);
-    double dT, dD, dB;
+    double xx, xy, yy;
     if (CUT) {
         const PiemdQ b = piemd_q_term(h, s, h.rcut, h.rb2);
-        dT = __builtin_fma(a.Tn, a.iq, -(b.Tn * b.iq));
-        dD = __builtin_fma(a.Dn, a.iq, -(b.Dn * b.iq));
-        dB = __builtin_fma(a.Bn, a.iq, -(b.Bn * b.iq));
+        xx = __builtin_fma(a.xx, a.iq, -(b.xx * b.iq));
+        xy = __builtin_fma(a.xy, a.iq, -(b.xy * b.iq));
+        yy = __builtin_fma(a.yy, a.iq, -(b.yy * b.iq));
     } else {
-        dT = a.Tn * a.iq;
-        dD = a.Dn * a.iq;
-        dB = a.Bn * a.iq;
+        xx = a.xx * a.iq;
+        xy = a.xy * a.iq;
+        yy = a.yy * a.iq;
     }
+    const double dT = xx + yy, dD = xx - yy, dB = xy;
     aT = __builtin_fma(h.qT, dT, aT);
     aD = __builtin_fma(h.qc2, dD, __builtin_fma(h.qs2m2, dB, aD));
     aB = __builtin_fma(h.qs2h, dD, __builtin_fma(h.qc2, dB, aB));
