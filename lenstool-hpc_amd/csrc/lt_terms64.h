@@ -197,58 +197,73 @@ __device__ __forceinline__ void grad_sis(const HaloRec& h, double px, double py,
 // ------------------------------------------------------------------------------------------------
 // potential (complex I_{0.5} potential of Kassiola & Kovner, src/potential_GPU.cu:47-72)
 // ------------------------------------------------------------------------------------------------
-struct cplx {
-    double re, im;
+// The reference's I_{0.5} potential (src/potential_GPU.cu:47-72) is written with complex
+// cosh / sinh / log of eta = -asinh(e1 y / z) / 2 + i asin(e2 x / z) / 2 and the real
+// zeta = ln((rem + sqrt(rc^2 + rem^2)) / rc) / 2. Every hyperbolic and circular function of eta and
+// zeta it uses has an algebraic value: with p = e1 y / z, q = e2 x / z, P = sqrt(1 + p^2),
+// C = sqrt(1 - q^2) and W = sqrt(rc^2 + rem^2),
+//   cosh 2 eta = P C - i p q,   sinh 2 eta = -p C + i P q,   tanh eta = sinh 2 eta / (1 + cosh 2 eta),
+//   sinh 2 zeta = rem / rc,     sinh^2 zeta = (W / rc - 1) / 2,   tanh zeta = rem / (rc + W),
+// so that (identities cosh(a+b) cosh(a-b) = cosh^2 a + sinh^2 b and cosh(a+b) / cosh(a-b) =
+// (1 + tanh a tanh b) / (1 - tanh a tanh b))
+//   ln(cosh^2 eta / (cosh(eta+zeta) cosh(eta-zeta))) = ln((1 + P C - i p q) / (P C + W / rc - i p q))
+//   ln(cosh(eta+zeta) / cosh(eta-zeta))             = ln((D + t N) / (D - t N)),
+// N = -p C + i P q, D = 1 + P C - i p q, t = tanh zeta. Two logarithms and two arguments per core
+// radius instead of five complex cosh / sinh, two complex logarithms, asinh and asin. The parts that
+// depend only on the pixel and the ellipticity are shared by the core and cut radii.
+struct PotShared {
+    double p, q, P, C, rem2;
 };
-__device__ __forceinline__ cplx cmul(cplx a, cplx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
-__device__ __forceinline__ cplx cdiv(cplx a, cplx b)
-{
-    double n = 1.0 / (b.re * b.re + b.im * b.im);
-    return {(a.re * b.re + a.im * b.im) * n, (a.im * b.re - a.re * b.im) * n};
-}
-__device__ __forceinline__ cplx ccosh(cplx a)
-{
-    double ch = cosh(a.re), sh = sinh(a.re), s, c;
-    sincos(a.im, &s, &c);
-    return {ch * c, sh * s};
-}
-__device__ __forceinline__ cplx csinh(cplx a)
-{
-    double ch = cosh(a.re), sh = sinh(a.re), s, c;
-    sincos(a.im, &s, &c);
-    return {sh * c, ch * s};
-}
-__device__ __forceinline__ cplx clog(cplx a) { return {0.5 * log(a.re * a.re + a.im * a.im), atan2(a.im, a.re)}; }
 
-__device__ inline double pi05(double x, double y, double eps, double rc, double b0)
+__device__ __forceinline__ PotShared pot_shared(double x, double y, double eps)
 {
-    double sqe = sqrt(eps);
-    double ci = .5 * (1. - eps * eps) / sqe;
-    double rem2 = x * x / ((1. + eps) * (1. + eps)) + y * y / ((1. - eps) * (1. - eps));
-    double e1 = 2. * sqe / (1 - eps);
-    double e2 = 2. * sqe / (1 + eps);
-    double z = sqrt(x * x + y * y);
-    cplx eta = {-.5 * asinh(e1 * y / z), .5 * asin(e2 * x / z)};
-    double srem = sqrt(rem2);
-    cplx zeta = {0.5 * log((srem + sqrt(rc * rc + rem2)) / rc), 0.};
-    cplx b1 = ccosh({eta.re + zeta.re, eta.im});
-    cplx b2 = ccosh({eta.re - zeta.re, eta.im});
-    cplx ce = ccosh(eta);
-    cplx a1 = clog(cdiv(cmul(ce, ce), cmul(b1, b2)));
-    cplx a2 = clog(cdiv(b1, b2));
-    cplx c1 = cmul(csinh({2. * eta.re, 2. * eta.im}), a1);
-    cplx c2 = cmul(csinh({2. * zeta.re, 0.}), a2);
-    return b0 * ci * rc / srem * ((c1.im + c2.im) * x - (c1.re + c2.re) * y);
+    const double sqe = sqrt(eps);
+    const double iz = rsq64(__builtin_fma(x, x, y * y));
+    PotShared s;
+    s.p = 2. * sqe / (1. - eps) * y * iz;
+    s.q = 2. * sqe / (1. + eps) * x * iz;
+    s.P = sqrt(__builtin_fma(s.p, s.p, 1.));
+    s.C = sqrt(__builtin_fma(-s.q, s.q, 1.));
+    s.rem2 = x * x / ((1. + eps) * (1. + eps)) + y * y / ((1. - eps) * (1. - eps));
+    return s;
+}
+
+// ln(u / v) and arg(u / v) of two complex numbers: one logarithm, one atan2
+__device__ __forceinline__ void log_ratio(double ur, double ui, double vr, double vi, double& lr, double& li)
+{
+    lr = 0.5 * log(__builtin_fma(ur, ur, ui * ui) / __builtin_fma(vr, vr, vi * vi));
+    li = atan2(__builtin_fma(ui, vr, -(ur * vi)), __builtin_fma(ur, vr, ui * vi));
+}
+
+// pi05 of one core radius without the amplitude b0 ci: rc / rem ((c1 + c2).im x - (c1 + c2).re y)
+__device__ __forceinline__ double pi05_core(const PotShared& s, double x, double y, double rc)
+{
+    const double srem = sqrt(s.rem2);
+    const double W = sqrt(__builtin_fma(rc, rc, s.rem2));
+    const double PC = s.P * s.C, pq = s.p * s.q;
+    double a1r, a1i, a2r, a2i;
+    log_ratio(1. + PC, -pq, PC + W / rc, -pq, a1r, a1i);
+    const double t = srem / (rc + W);
+    const double Nr = -s.p * s.C, Ni = s.P * s.q;
+    log_ratio(__builtin_fma(t, Nr, 1. + PC), __builtin_fma(t, Ni, -pq), __builtin_fma(-t, Nr, 1. + PC),
+              __builtin_fma(-t, Ni, -pq), a2r, a2i);
+    // c1 = sinh(2 eta) a1 with sinh 2 eta = N; c2 = (rem / rc) a2
+    const double sz = srem / rc;
+    const double cr = __builtin_fma(Nr, a1r, -(Ni * a1i)) + sz * a2r;
+    const double ci = __builtin_fma(Nr, a1i, Ni * a1r) + sz * a2i;
+    return rc / srem * __builtin_fma(ci, x, -(cr * y));
 }
 
 __device__ __forceinline__ double pot_halo(const HaloRec& h, int type, double px, double py)
 {
     PixelFrame f = halo_frame(h, px, py);
     if (type == TYPE_SIS) return h.b0 * sqrt(__builtin_fma(h.sa, f.x2, h.sb * f.y2));
-    double pa = pi05(f.x, f.y, h.eps, h.rc, h.b0);
-    if (type == TYPE_PIEMD) return pa;
-    double ps = pi05(f.x, f.y, h.eps, h.rcut, h.b0);
-    return (h.rcut / (h.rcut - h.rc)) * (pa - ps);
+    const double amp = h.b0 * 0.5 * (1. - h.eps * h.eps) / sqrt(h.eps);
+    const PotShared s = pot_shared(f.x, f.y, h.eps);
+    const double pa = pi05_core(s, f.x, f.y, h.rc);
+    if (type == TYPE_PIEMD) return amp * pa;
+    const double ps = pi05_core(s, f.x, f.y, h.rcut);
+    return (h.rcut / (h.rcut - h.rc)) * amp * (pa - ps);
 }
 
 // ------------------------------------------------------------------------------------------------
