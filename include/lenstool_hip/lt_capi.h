@@ -208,6 +208,37 @@ int lt_server_reset_stats(lt_server server);
 /* Stops the dispatcher, waits for the device and frees everything (unfetched results included). */
 int lt_server_destroy(lt_server server);
 
+/* ---- Lenstool parameter files ---------------------------------------------------------------------
+ * Reads a .par file and the catalogues it names (potfiles, multiple-image file) the way the
+ * reference's readers do (src/module_readParameters.cpp: cosmology :278-384, runmode :385-851,
+ * potentials :2234-2489, potfiles :1102-1656, frame :2954-3030, images :852-935): potentials slotted by
+ * ascending type, then potfile clumps with their scaling relations; kpc radii converted with the
+ * file's cosmology. Catalogue paths are taken as written, else relative to the .par's directory. */
+typedef struct lt_par_s* lt_par;
+typedef struct lt_par_info {
+    int nhalos;               /* potentials + potfile clumps                                  */
+    int nhalos_main;          /* potentiel blocks                                             */
+    int npotfiles;
+    int nbgridcells;          /* runmode nbgridcells (default 1000)                            */
+    int ampli, ampli_cells;  double ampli_z;   /* runmode ampli  <mode> <cells> <z>          */
+    int mass, mass_cells;    double mass_z;    /* runmode mass                                */
+    int shear, shear_cells;  double shear_z;   /* runmode shear                               */
+    int poten, poten_cells;  double poten_z;   /* runmode poten                               */
+    int dpl, dpl_cells;      double dpl_z;     /* runmode dpl                                 */
+    int inverse;
+    int ref_mode;  double ref_ra, ref_dec;     /* runmode reference                           */
+    double z_source;          /* source block z_source (default 0.8); the dlsds of the model    */
+    int nsets, nimagestot;    /* multiple-image families and images                           */
+} lt_par_info;
+int lt_par_load(lt_par* out, const char* path);
+/* Run settings, the lens model (arrays owned by the parameter set, valid until lt_par_destroy;
+ * type, position, b0, vdisp, angle, ellipticity, epot, rcore, rcut, z, mag, lum, theta, cos/sin,
+ * dlsds filled), the frame and the cosmology. Any pointer may be NULL. */
+int lt_par_get(lt_par par, lt_par_info* info, lt_potential_soa* lens, lt_grid_param* frame, lt_cosmo_param* cosmo);
+/* The multiple images (arrays of nsets and nimagestot elements, caller-allocated; NULL skips). */
+int lt_par_images(lt_par par, int* nimages_per_set, lt_galaxy* images);
+int lt_par_destroy(lt_par par);
+
 /* ---- point evaluations, delensing and chi^2 ------------------------------------------------------ */
 
 /* Gradient at npts points (host arrays). The batched counterpart of the reference's point evaluator

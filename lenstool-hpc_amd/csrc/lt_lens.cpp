@@ -86,7 +86,7 @@ int pack_model(const lt_potential_soa* L, int n, Use use, std::vector<HaloRec>& 
             const double rc = L->rcore[i];
             const double rcut = (t == TYPE_DPIE) ? L->rcut[i] : 0.0;
             if (!(eps > 0.0 && eps < 1.0)) return bad(err, i, "PIEMD ellipticity_potential must be in (0, 1)");
-            if (!(rc > 0.0) || !finite(rc)) return bad(err, i, "PIEMD rcore must be > 0");
+            if (!(rc >= 0.0) || !finite(rc)) return bad(err, i, "PIEMD rcore must be >= 0");
             if (t == TYPE_DPIE && (!finite(rcut) || !(rcut > 0.0) || rcut == rc))
                 return bad(err, i, "dPIE rcut must be > 0 and differ from rcore");
             const double sqe = std::sqrt(eps);

@@ -51,6 +51,17 @@ class LtServerStats(C.Structure):
                 ("elapsed_ms", C.c_double), ("bytes_in_use", C.c_longlong), ("bytes_peak", C.c_longlong)]
 
 
+class LtParInfo(C.Structure):
+    _fields_ = [("nhalos", C.c_int), ("nhalos_main", C.c_int), ("npotfiles", C.c_int), ("nbgridcells", C.c_int),
+                ("ampli", C.c_int), ("ampli_cells", C.c_int), ("ampli_z", C.c_double),
+                ("mass", C.c_int), ("mass_cells", C.c_int), ("mass_z", C.c_double),
+                ("shear", C.c_int), ("shear_cells", C.c_int), ("shear_z", C.c_double),
+                ("poten", C.c_int), ("poten_cells", C.c_int), ("poten_z", C.c_double),
+                ("dpl", C.c_int), ("dpl_cells", C.c_int), ("dpl_z", C.c_double),
+                ("inverse", C.c_int), ("ref_mode", C.c_int), ("ref_ra", C.c_double), ("ref_dec", C.c_double),
+                ("z_source", C.c_double), ("nsets", C.c_int), ("nimagestot", C.c_int)]
+
+
 class LtError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__(f"lenstool_hip status {status}: {msg}")
@@ -109,6 +120,11 @@ SIGNATURES = {
     "lt_server_destroy": ([C.c_void_p], C.c_int),
     "lt_autotune": ([C.c_void_p, C.c_int, C.c_int, C.POINTER(LtGridDesc), C.c_int, C.c_void_p,
                      C.POINTER(C.c_int), C.POINTER(C.c_float)], C.c_int),
+    "lt_par_load": ([C.POINTER(C.c_void_p), C.c_char_p], C.c_int),
+    "lt_par_get": ([C.c_void_p, C.POINTER(LtParInfo), C.POINTER(M.LtPotentialSOA), C.POINTER(M.LtGridParam),
+                    C.POINTER(M.LtCosmoParam)], C.c_int),
+    "lt_par_images": ([C.c_void_p, C.POINTER(C.c_int), C.POINTER(M.LtGalaxy)], C.c_int),
+    "lt_par_destroy": ([C.c_void_p], C.c_int),
     "lt_graph_begin": ([C.c_void_p], C.c_int),
     "lt_graph_end": ([C.c_void_p, C.POINTER(C.c_void_p)], C.c_int),
     "lt_graph_launch": ([C.c_void_p, C.c_void_p], C.c_int),
@@ -441,6 +457,33 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def load_par(path):
+    """Read a Lenstool .par (and its potfiles / image file) with the library's reader. Returns a dict
+    with `info` (run settings), `lens` (LensModel, halos in the reference's SOA order), `frame`,
+    `cosmo`, `nimages` and `images` (GALAXY_DTYPE array)."""
+    h = C.c_void_p()
+    check(lib().lt_par_load(C.byref(h), str(path).encode()))
+    try:
+        info, soa, frame, cosmo = LtParInfo(), M.LtPotentialSOA(), M.LtGridParam(), M.LtCosmoParam()
+        check(lib().lt_par_get(h, C.byref(info), C.byref(soa), C.byref(frame), C.byref(cosmo)))
+        n = info.nhalos
+
+        def arr(ptr, dtype=np.float64):
+            return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype).copy() if n else np.zeros(0, dtype)
+        lens = M.LensModel(type=arr(soa.type, np.int32), x=arr(soa.position_x), y=arr(soa.position_y),
+                           b0=arr(soa.b0), angle=arr(soa.ellipticity_angle), epot=arr(soa.ellipticity_potential),
+                           rcore=arr(soa.rcore), rcut=arr(soa.rcut), z=arr(soa.z), dlsds=arr(soa.dlsds))
+        extra = {"vdisp": arr(soa.vdisp), "ellipticity": arr(soa.ellipticity), "mag": arr(soa.mag),
+                 "anglecos": arr(soa.anglecos), "anglesin": arr(soa.anglesin)}
+        nimages = np.zeros(info.nsets, dtype=np.int32)
+        images = M.galaxies_array(info.nimagestot)
+        check(lib().lt_par_images(h, nimages.ctypes.data_as(C.POINTER(C.c_int)), M.galaxy_ptr(images)))
+        return {"info": {f: getattr(info, f) for f, _ in LtParInfo._fields_}, "lens": lens, "extra": extra,
+                "frame": frame, "cosmo": cosmo, "nimages": nimages, "images": images}
+    finally:
+        lib().lt_par_destroy(h)
 
 
 def stream_create() -> int:

@@ -296,3 +296,35 @@ def test_config2_fullsize_sampled(hip, oracle):
     sa, sb, sc, sd = hip.hessian_grid(lens, sub, 1500, 2100)
     np.testing.assert_array_equal(sa, a[2100:2300, 1500:1800])
     np.testing.assert_array_equal(sc, c[2100:2300, 1500:1800])
+
+
+# ------------------------------------------------------------------------------------------------
+# a real model: the A2744 parameter file read by the library's reader (258 dPIE halos)
+# ------------------------------------------------------------------------------------------------
+
+def test_real_model_a2744(hip, oracle):
+    """Hessian and gradient of the 258-halo A2744 model (6 cluster-scale dPIE + 252 potfile galaxies)
+    on a 200x160 grid over the cluster core, against the oracle with the FP64 criterion, and the
+    FP32C det A within 1e-6 of FP64."""
+    import os
+    from conftest import GOLDEN
+    d = hip.load_par(os.path.join(GOLDEN, "par", "A2744_SLspec.par"))
+    lens = d["lens"]
+    grid = M.Grid(M.grid_param(-60.0, 60.0, -50.0, 46.0), 200, 160)
+    print_case("A2744 real model", lens, grid)
+    a, b, c, _ = hip.hessian_grid(lens, grid)
+    ea, eb, ec, _ = oracle.hessian_planes(lens, grid, extended=True)
+    _, _, _, _, aa, ab, ac = oracle.hessian_planes(lens, grid, with_abs=True)
+    _, ch = oracle.cond_planes(lens, grid)
+    for got, ex, sc in ((a, ea, aa), (b, eb, ab), (c, ec, ac)):
+        assert (np.abs(got - ex) <= REL * sc + ch).all()
+    gx, gy = hip.gradient_grid(lens, grid)
+    ex, ey = oracle.gradient_grid(lens, grid, extended=True)
+    _, _, ax, ay = oracle.gradient_grid(lens, grid, with_abs=True)
+    cg, _ = oracle.cond_planes(lens, grid)
+    assert (np.abs(gx - ex) <= REL * ax + cg).all() and (np.abs(gy - ey) <= REL * ay + cg).all()
+    r = hip.cosmo_dlsds(0.308, 2.0, d["cosmo"])
+    f64 = hip.map_fields(lens, grid, d["cosmo"], 0.308, 2.0)
+    f32 = hip.map_fields(lens, grid, d["cosmo"], 0.308, 2.0, precision=hip.PREC_FP32C)
+    assert np.abs(f32[3] - f64[3]).max() <= 1e-6
+    assert r > 0
