@@ -170,3 +170,13 @@ void mychi_bruteforce_SOA_CPU_grid_gradient_barycentersource(double* chi, int* e
                                                              const struct Potential_SOA* lens,
                                                              const struct grid_param* frame,
                                                              const int* nimages_strongLensing, galaxy* images);
+
+// ---- FITS maps (src/module_writeFits.hpp): float32 images with the frame's world coordinates ----
+int module_writeFits_Image(char* filename, double* ima, int nx, int ny, double xmin, double xmax, double ymin,
+                           double ymax);
+int module_writeFits_ImageAbsoluteCoordinates(char* filename, double* ima, int nx, int ny, double xmin, double xmax,
+                                              double ymin, double ymax, double ra, double dec);
+void module_writeFits(std::string path, std::string filename, int ii, type_t* map, const struct runmode_param* runmode,
+                      int size, const struct grid_param* frame, type_t ra, type_t dec);
+void module_writeFits(std::string path, std::string filename, type_t* map, const struct runmode_param* runmode,
+                      int size, const struct grid_param* frame, type_t ra, type_t dec);

@@ -141,6 +141,14 @@ typedef struct lt_grid_desc {
     long long pitch;
 } lt_grid_desc;
 
+/* The r (D_ls/D_s) and mult arguments of lt_map_device for a reference map kind and mode, formed
+ * as the reference's map wrappers form them (src/grid_map_ampli_GPU.cu:110-112,
+ * src/grid_map_mass_GPU.cu:104-106, 219-305, src/grid_map_pot_GPU.cu:100-102, 178-270): mass modes 2, 3, 4 scale by
+ * the critical density in g/cm^2, 1e12 Msun/pixel, 1e12 Msun/kpc^2; potential mode 1 by r,
+ * mode 2 by c^2 / (2 pi G) h / D_ol. dx, dy (arcsec) matter for mass mode 3 only. */
+int lt_map_factors(int kind, int mode, const lt_cosmo_param* cosmo, double z_lens, double z_source, double dx,
+                   double dy, double* r, double* mult);
+
 /* Compute one map into device arrays on `stream` (asynchronous). kind/mode as lt_map_grid plus
  * LT_MAP_HESSIAN (out0..out3 = a, b, c, d), LT_MAP_GRADIENT (out0, out1 = x, y) and LT_MAP_FIELDS
  * (out0..out4 = kappa, gamma1, gamma2, det A, mu). r = D_ls/D_s used by the map formulas, mult = the
@@ -237,7 +245,33 @@ int lt_par_load(lt_par* out, const char* path);
 int lt_par_get(lt_par par, lt_par_info* info, lt_potential_soa* lens, lt_grid_param* frame, lt_cosmo_param* cosmo);
 /* The multiple images (arrays of nsets and nimagestot elements, caller-allocated; NULL skips). */
 int lt_par_images(lt_par par, int* nimages_per_set, lt_galaxy* images);
+/* Reads a bayes.dat file of MCMC samples for this model (module_readParameters_preparebayes and
+ * _bayesmodels, src/module_readParameters.cpp:53-108; a path instead of ./bayes.dat): one column per
+ * '#' line, one sample per line. A line with more values than columns is LT_ERR_INVALID. */
+int lt_par_bayes_load(lt_par par, const char* path, int* nsamples, int* ncolumns);
+/* Sets the lens model to bayes sample `sample` (module_readParameters_setbayesmapmodels,
+ * src/module_readParameters.cpp:132-227): columns from index 2 fill the parameters each `limit`
+ * block frees (block >= 1), potential by potential in file order, in the order x, y, epot,
+ * ellipticity, angle [deg], rcore, vdisp, rcut, z; b0 and the ellipticity pair are recomputed; with
+ * potfiles, the image-redshift (z_m_limit) columns are skipped, then each potfile's cut [arcsec] and
+ * sigma (if freed) are read and its scaling relations re-applied. In place, like the reference
+ * (lt_par_get then returns the sample's model); D_ls/D_s is left as read. A sample with fewer
+ * values than the free parameters need is LT_ERR_INVALID and leaves the model unchanged. */
+int lt_par_bayes_apply(lt_par par, int sample);
+/* File name of a runmode map line (`ampli`, `shear`, `mass`, `poten`; `dpl` has two: which = 0 for
+ * x, 1 for y), as read by read_runmode (src/module_readParameters.cpp:415-465); "" if absent. */
+int lt_par_map_name(lt_par par, int kind, int which, char* buf, int cap);
 int lt_par_destroy(lt_par par);
+
+/* ---- FITS maps -------------------------------------------------------------------------------------
+ * Writes a host map (ny rows of nx doubles) as a FITS primary image of float32 values with the
+ * frame's world coordinates: absolute == 0 gives module_writeFits_Image's linear arcsec axes
+ * (pixel 1 at xmin / ymin, step (max - min) / (n - 1)), absolute != 0
+ * module_writeFits_ImageAbsoluteCoordinates' RA---TAN / DEC--TAN axes in degrees around (ra, dec)
+ * (src/module_writeFits.cpp:113-335; module_writeFits picks absolute unless ra == dec == 0).
+ * Overwrites `path`. nx, ny >= 2. */
+int lt_fits_write(const char* path, const double* map, int nx, int ny, double xmin, double xmax, double ymin,
+                  double ymax, int absolute, double ra, double dec);
 
 /* ---- point evaluations, delensing and chi^2 ------------------------------------------------------ */
 

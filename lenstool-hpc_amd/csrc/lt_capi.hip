@@ -478,19 +478,12 @@ int lt_potential_grid(double* phi, const lt_grid_param* frame, const lt_potentia
                     jstart, LT_PREC_FP64);
 }
 
-int lt_map_grid(int kind, int mode, double* map0, double* map1, const lt_cosmo_param* cosmo,
-                const lt_grid_param* frame, const lt_potential_soa* lens, int nhalos, double z_lens, double z_source,
-                double dx, double dy, int nx, int ny, int istart, int jstart, int precision)
+int lt_map_factors(int kind, int mode, const lt_cosmo_param* cosmo, double z_lens, double z_source, double dx,
+                   double dy, double* r_out, double* mult_out)
 {
     int st = check_kind_mode(kind, mode);
     if (st) return st;
-    if (kind == LT_MAP_FIELDS || kind == LT_MAP_HESSIAN || kind == LT_MAP_GRADIENT)
-        return fail(LT_ERR_INVALID, "lt_map_grid takes the reference map kinds AMPLI, SHEAR, MASS, POT, DPL");
-    if (!cosmo) return fail(LT_ERR_INVALID, "cosmo is NULL");
-    if (kind == LT_MAP_MASS && z_lens == 0.0) {
-        if (!lens || !lens->z || nhalos <= 0) return fail(LT_ERR_INVALID, "mass map: z_lens = 0 needs lens->z[0]");
-        z_lens = lens->z[0];
-    }
+    if (!cosmo || !r_out || !mult_out) return fail(LT_ERR_INVALID, "NULL argument");
     // distances as the reference map wrappers form them (src/grid_map_ampli_GPU.cu:110-112,
     // src/grid_map_mass_GPU.cu:101-108)
     const double dl0s = cosmo_object_object(z_lens, z_source, *cosmo);
@@ -509,6 +502,27 @@ int lt_map_grid(int kind, int mode, double* map0, double* map1, const lt_cosmo_p
     } else if (kind == LT_MAP_POT) {
         mult = mode == 1 ? r : mode == 2 ? LT_CH2PIG * h / dol : 1.0;
     }
+    *r_out = r;
+    *mult_out = mult;
+    return LT_OK;
+}
+
+int lt_map_grid(int kind, int mode, double* map0, double* map1, const lt_cosmo_param* cosmo,
+                const lt_grid_param* frame, const lt_potential_soa* lens, int nhalos, double z_lens, double z_source,
+                double dx, double dy, int nx, int ny, int istart, int jstart, int precision)
+{
+    int st = check_kind_mode(kind, mode);
+    if (st) return st;
+    if (kind == LT_MAP_FIELDS || kind == LT_MAP_HESSIAN || kind == LT_MAP_GRADIENT)
+        return fail(LT_ERR_INVALID, "lt_map_grid takes the reference map kinds AMPLI, SHEAR, MASS, POT, DPL");
+    if (!cosmo) return fail(LT_ERR_INVALID, "cosmo is NULL");
+    if (kind == LT_MAP_MASS && z_lens == 0.0) {
+        if (!lens || !lens->z || nhalos <= 0) return fail(LT_ERR_INVALID, "mass map: z_lens = 0 needs lens->z[0]");
+        z_lens = lens->z[0];
+    }
+    double r = 0, mult = 1;
+    st = lt_map_factors(kind, mode, cosmo, z_lens, z_source, dx, dy, &r, &mult);
+    if (st) return st;
     double* outs[5] = {map0, kind == LT_MAP_DPL ? map1 : nullptr, nullptr, nullptr, nullptr};
     Use use = kind == LT_MAP_DPL ? Use::Gradient : kind == LT_MAP_POT ? Use::Potential
               : kind == LT_MAP_MASS ? Use::Kmap : Use::Hessian;

@@ -10,6 +10,8 @@
 //   images               src/module_readParameters.cpp:852-935
 //   dlsds                src/module_readParameters.cpp:230-257
 //   WCS round trip       src/convert_coordinates.cpp
+//   limit blocks         src/module_readParameters.cpp:1734-1937
+//   bayes.dat            src/module_readParameters.cpp:53-108 (reading), 132-227 (column mapping)
 // Reference behaviour kept on purpose: potentials are slotted by ascending type (file order within
 // a type) and potfile clumps follow them in catalogue order; a flat universe (omegaK = 0) takes
 // omegaX = 1 - omegaM whatever `lambda` says; a PIEMD with zero ellipticity gets epot = 1e-5; potfile
@@ -45,6 +47,14 @@ struct Potfile {
     std::string file;
     double zlens = 0, mag0 = 0, sigma1 = 0, core = -1.0, corekpc = -1.0;
     double cut1 = DBL_MAX, cutkpc1 = DBL_MAX, slope1 = 0, vdslope1 = 0;
+    int first = 0, count = 0;  // clump slots in the lens model
+};
+
+// the parameters a limit block can free, in the order the bayes.dat columns of one potential
+// follow (src/module_readParameters.cpp:139-188)
+enum LimitParam { LIM_X, LIM_Y, LIM_EPOT, LIM_ELL, LIM_ANGLE, LIM_RCORE, LIM_VDISP, LIM_RCUT, LIM_Z, LIM_N };
+struct Limit {
+    int block[LIM_N] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 std::string tok(const std::string& line, int k)
@@ -98,26 +108,6 @@ void potential_parameters(Halo& h)
     }
 }
 
-// the same for a potfile clump (src/module_readParameters.cpp:2873-2944: the SIS branch differs)
-void clump_parameters(Halo& h)
-{
-    switch (h.type) {
-    case 5:
-        h.b0 = 4. * LT_PI_C2 * h.vdisp * h.vdisp;
-        if (h.ell == 0. && h.epot != 0.) h.epot = h.ell / 3.;
-        else h.ell = h.epot * 3.;
-        break;
-    case 8:
-    case 81:
-        potential_parameters(h);
-        break;
-    default:
-        if (h.ell == 0. && h.epot != 0.) h.ell = h.epot * 3.;
-        else h.epot = h.ell / 3.;
-        break;
-    }
-}
-
 // relative arcsec -> absolute degrees -> relative arcsec (src/convert_coordinates.cpp)
 void wcs_round_trip(double& x, double& y, int mode, double ra, double dec)
 {
@@ -150,6 +140,16 @@ struct lt_par_s {
     std::vector<int> nimages;
     std::vector<lt_galaxy> images;
     std::string imagefile;
+    // optimisation state: limit blocks and lens-model slots of the potentials in file order, the
+    // potfiles' scaling state, the number of image-redshift parameters, the bayes.dat rows
+    std::vector<Limit> limits;
+    std::vector<int> main_slot;
+    std::vector<Potfile> pots;
+    int n_z_param = 0;
+    int bayes_ncols = 0;
+    std::vector<std::vector<double>> bayes;
+    // runmode map file names: ampli, shear, mass, poten, dpl x, dpl y
+    std::string names[6];
 
     void push(const Halo& h)
     {
@@ -176,6 +176,49 @@ namespace {
 
 // kpc per arcsec at redshift z: d0 / h * D_A(z)
 double kpc_per_arcsec(double z, const lt_cosmo_param& c) { return LT_D0 / c.h * cosmo_observer_object(z, c); }
+
+// b0 and the ellipticity pair of lens-model slot i after a change of its parameters
+// (module_readParameters_calculatePotentialparameter_SOA, src/module_readParameters.cpp:2873-2944;
+// unlike the reader of potentiel blocks, the SIS branch keeps the potential ellipticity)
+void soa_parameters(lt_par_s& P, size_t i)
+{
+    double& ell = P.ell[i];
+    double& epot = P.epot[i];
+    switch (P.type[i]) {
+    case 5:
+        P.b0[i] = 4. * LT_PI_C2 * P.vdisp[i] * P.vdisp[i];
+        if (ell == 0. && epot != 0.) epot = ell / 3.;
+        else ell = epot * 3.;
+        break;
+    case 8:
+    case 81:
+        P.b0[i] = 6. * LT_PI_C2 * P.vdisp[i] * P.vdisp[i];
+        if (ell == 0. && epot != 0.) ell = 2. * epot / (1. + epot * epot);
+        else if (ell == 0. && epot == 0.) epot = 0.00001;
+        else epot = (1. - std::sqrt(1 - ell * ell)) / ell;
+        break;
+    default:
+        if (ell == 0. && epot != 0.) ell = epot * 3.;
+        else epot = ell / 3.;
+        break;
+    }
+}
+
+// the scaling relations of a potfile applied to its clumps (setScalingRelations,
+// src/module_readParameters.cpp:1539-1625): core radius, velocity dispersion and cut radius from
+// the clump magnitude; `core` and `cut1` are in arcsec by then
+void scale_potfile(lt_par_s& P, const Potfile& pf)
+{
+    for (int i = pf.first; i < pf.first + pf.count; ++i) {
+        const double dm = 0.4 * (pf.mag0 - P.mag[i]);
+        if (P.mag[i] != 0) {
+            P.rcore[i] = pf.core * std::pow(10., dm / 2.);
+            P.vdisp[i] = pf.sigma1 * std::pow(10., dm / pf.vdslope1);
+            P.rcut[i] = pf.cut1 * std::pow(10., dm * 2. / pf.slope1);
+        }
+        soa_parameters(P, (size_t)i);
+    }
+}
 
 int parse(lt_par_s& P, const std::string& path)
 {
@@ -205,7 +248,7 @@ int parse(lt_par_s& P, const std::string& path)
     R.z_source = 0.8;
 
     std::vector<Halo> main;
-    std::vector<Potfile> pots;
+    std::vector<Potfile>& pots = P.pots;
     // block end: a line whose first word starts with "end"
     auto block_end = [&](size_t i) { return starts(tok(L[i], 0), "end"); };
     // the cosmology first, wherever it stands in the file: kpc radii of every block depend on it
@@ -239,25 +282,31 @@ int parse(lt_par_s& P, const std::string& path)
             for (++i; i < L.size() && !block_end(i); ++i) {
                 const std::string& s = L[i];
                 const std::string k = tok(s, 0);
-                char buf[4096];
+                char buf[4096] = "", buf2[4096] = "";
                 double v = 0;
                 if (k == "nbgridcells") std::sscanf(s.c_str(), " %*s %d", &R.nbgridcells);
                 else if (k == "inverse") std::sscanf(s.c_str(), " %*s %d", &R.inverse);
                 else if (k == "ampli") {
                     std::sscanf(s.c_str(), " %*s %d %d %lf %4095s", &R.ampli, &R.ampli_cells, &v, buf);
                     R.ampli_z = v;
+                    P.names[0] = buf;
                 } else if (k == "mass") {
                     std::sscanf(s.c_str(), " %*s %d %d %lf %4095s", &R.mass, &R.mass_cells, &v, buf);
                     R.mass_z = v;
+                    P.names[2] = buf;
                 } else if (k == "shear") {
                     std::sscanf(s.c_str(), " %*s %d %d %lf %4095s", &R.shear, &R.shear_cells, &v, buf);
                     R.shear_z = v;
+                    P.names[1] = buf;
                 } else if (k == "poten") {
                     std::sscanf(s.c_str(), " %*s %d %d %lf %4095s", &R.poten, &R.poten_cells, &v, buf);
                     R.poten_z = v;
+                    P.names[3] = buf;
                 } else if (k == "dpl") {
-                    std::sscanf(s.c_str(), " %*s %d %d %lf", &R.dpl, &R.dpl_cells, &v);
+                    std::sscanf(s.c_str(), " %*s %d %d %lf %4095s %4095s", &R.dpl, &R.dpl_cells, &v, buf, buf2);
                     R.dpl_z = v;
+                    P.names[4] = buf;
+                    P.names[5] = buf2;
                 } else if (k == "reference") {
                     double a = 0, b = 0;
                     std::sscanf(s.c_str(), " %*s %d %lf %lf", &R.ref_mode, &a, &b);
@@ -277,6 +326,8 @@ int parse(lt_par_s& P, const std::string& path)
                     int on = 0;
                     char buf[4096];
                     if (std::sscanf(L[i].c_str(), " %*s %d %4095s", &on, buf) == 2 && on >= 1) P.imagefile = buf;
+                } else if (tok(L[i], 0) == "z_m_limit") {
+                    P.n_z_param += 1;  // one bayes.dat column per image-family redshift
                 }
             }
         } else if (starts(first, "grille") || starts(first, "frame") || starts(first, "champ")) {
@@ -325,6 +376,25 @@ int parse(lt_par_s& P, const std::string& path)
                 }
             }
             pots.push_back(pf);
+        } else if (starts(first, "limit")) {
+            // the k-th limit block frees parameters of the k-th potentiel block
+            Limit lim;
+            for (++i; i < L.size() && !block_end(i); ++i) {
+                const std::string k = tok(L[i], 0);
+                int block = 0;
+                std::sscanf(L[i].c_str(), " %*s %d", &block);
+                if (k == "x_centre" || k == "x_center") lim.block[LIM_X] = block;
+                else if (k == "y_centre" || k == "y_center") lim.block[LIM_Y] = block;
+                else if (k == "v_disp") lim.block[LIM_VDISP] = block;
+                else if (k == "ellip_pot") lim.block[LIM_EPOT] = block;
+                else if (k == "ellipticitymass" || k == "ellipticity" || k == "ellipticite" || k == "gamma")
+                    lim.block[LIM_ELL] = block;
+                else if (k == "ellipticity_angle" || k == "angle_pos") lim.block[LIM_ANGLE] = block;
+                else if (k == "rcut" || k == "cut_radius" || k == "cut_radius_kpc") lim.block[LIM_RCUT] = block;
+                else if (k == "rcore" || k == "core_radius" || k == "core_radius_kpc") lim.block[LIM_RCORE] = block;
+                else if (k == "z_lens") lim.block[LIM_Z] = block;
+            }
+            P.limits.push_back(lim);
         } else if (starts(first, "potent")) {
             Halo h;
             if (!main.empty()) h.type = main.back().type;  // a block without `profil` keeps the last one
@@ -356,9 +426,13 @@ int parse(lt_par_s& P, const std::string& path)
     R.nhalos_main = (int)main.size();
 
     // slot the potentials by ascending type, file order within a type
+    P.main_slot.assign(main.size(), 0);
     for (int t = 0; t < 100; ++t)
-        for (const Halo& h : main)
-            if (h.type == t) P.push(h);
+        for (size_t k = 0; k < main.size(); ++k)
+            if (main[k].type == t) {
+                P.main_slot[k] = (int)P.type.size();
+                P.push(main[k]);
+            }
     for (const Halo& h : main)
         if (h.type < 0 || h.type >= 100)
             return set_error(LT_ERR_TYPE, "potential profile " + std::to_string(h.type) + " out of range");
@@ -370,13 +444,17 @@ int parse(lt_par_s& P, const std::string& path)
             return set_error(LT_ERR_INVALID, "potfile " + pf.file + ": only catalogue types 1 and 3 are read");
         const std::string fn = resolve(pf.file, dir);
         if (fn.empty()) return set_error(LT_ERR_NOT_FOUND, "potfile " + pf.file + " not found");
+        if (pf.ftype == 1 && pf.cut1 == DBL_MAX && pf.cutkpc1 == DBL_MAX)
+            return set_error(LT_ERR_INVALID, "potfile " + pf.file + ": cut length not defined");
+        if (pf.ftype == 1 && pf.core == -1 && pf.corekpc == -1)
+            return set_error(LT_ERR_INVALID, "potfile " + pf.file + ": core length not defined");
         std::ifstream cat(fn.c_str());
         const double d1 = kpc_per_arcsec(pf.zlens, C);
-        const double cut = pf.cut1 == DBL_MAX && pf.cutkpc1 != DBL_MAX ? pf.cutkpc1 / d1 : pf.cut1;
-        const double core = pf.core == -1.0 && pf.corekpc != -1 ? pf.corekpc / d1 : pf.core;
+        if (pf.cut1 == DBL_MAX && pf.cutkpc1 != DBL_MAX) pf.cut1 = pf.cutkpc1 / d1;
+        if (pf.core == -1.0 && pf.corekpc != -1) pf.core = pf.corekpc / d1;
+        pf.first = (int)P.type.size();
         int ref_mode = 0;
         double ref_ra = 0, ref_dec = 0;
-        int n = 0;
         for (std::string s; std::getline(cat, s);) {
             if (starts(s, "#REFERENCE")) {
                 std::sscanf(s.c_str(), " %*s %d%lf%lf", &ref_mode, &ref_ra, &ref_dec);
@@ -398,18 +476,11 @@ int parse(lt_par_s& P, const std::string& path)
             h.lum = lm;
             h.mag = mg;
             h.angle = th * dtr;
-            // scaling relations (src/module_readParameters.cpp:1609-1625)
-            if (h.mag != 0) {
-                h.rcore = core * std::pow(10., 0.4 * (pf.mag0 - h.mag) / 2.);
-                h.vdisp = pf.sigma1 * std::pow(10., 0.4 * (pf.mag0 - h.mag) / pf.vdslope1);
-                h.rcut = cut * std::pow(10., 0.4 * (pf.mag0 - h.mag) * 2. / pf.slope1);
-            }
-            clump_parameters(h);
             P.push(h);
-            ++n;
+            pf.count += 1;
         }
+        scale_potfile(P, pf);
         R.npotfiles += 1;
-        (void)n;
     }
     R.nhalos = (int)P.type.size();
 
@@ -447,6 +518,103 @@ int parse(lt_par_s& P, const std::string& path)
         }
         R.nsets = (int)P.nimages.size();
         R.nimagestot = (int)P.images.size();
+    }
+    return LT_OK;
+}
+
+// bayes.dat: one column per '#' line, one sample per other line (src/module_readParameters.cpp:53-108).
+// Blank lines are skipped; a line with more values than columns is an error (the reference's
+// stream would carry the surplus into the next sample), a line with fewer keeps its count so
+// that the mapping can refuse a column that is absent.
+int bayes_read(lt_par_s& P, const std::string& path)
+{
+    std::ifstream in(path.c_str());
+    if (!in) return set_error(LT_ERR_NOT_FOUND, "bayes file " + path + " not found");
+    int ncols = 0;
+    std::vector<std::vector<double>> rows;
+    int line = 0;
+    for (std::string s; std::getline(in, s);) {
+        ++line;
+        if (starts(s, "#")) {
+            ncols += 1;
+            continue;
+        }
+        std::istringstream ss(s);
+        std::vector<double> row;
+        std::string w;
+        while (ss >> w) {
+            char* end = nullptr;
+            const double v = std::strtod(w.c_str(), &end);
+            if (end == w.c_str() || *end != '\0')
+                return set_error(LT_ERR_INVALID, path + ":" + std::to_string(line) + ": not a number: " + w);
+            row.push_back(v);
+        }
+        if (row.empty()) continue;
+        if ((int)row.size() > ncols)
+            return set_error(LT_ERR_INVALID, path + ":" + std::to_string(line) + ": " + std::to_string(row.size()) +
+                                                 " values for " + std::to_string(ncols) + " columns");
+        rows.push_back(std::move(row));
+    }
+    P.bayes_ncols = ncols;
+    P.bayes.swap(rows);
+    return LT_OK;
+}
+
+// the lens model of bayes sample k (module_readParameters_setbayesmapmodels,
+// src/module_readParameters.cpp:132-227): from column 2 on, the freed parameters of each potential
+// in file order (x, y, epot, ellipticity, angle [deg], rcore, vdisp, rcut, z), b0 and the
+// ellipticity pair recomputed; with potfiles, the image-redshift columns skipped and then each
+// potfile's cut radius [arcsec] and sigma if freed, its scaling relations re-applied. In place, as
+// the reference: parameters not in the file keep their values, D_ls/D_s is not recomputed.
+int bayes_apply(lt_par_s& P, int k)
+{
+    if (k < 0 || k >= (int)P.bayes.size())
+        return set_error(LT_ERR_INVALID, "bayes sample " + std::to_string(k) + " of " + std::to_string(P.bayes.size()));
+    const std::vector<double>& row = P.bayes[(size_t)k];
+    const double dtr = std::acos(-1.) / 180.;
+    size_t col = 2;
+    auto next = [&]() { return row[col++]; };
+    // every needed column must be there (checked first: a refused sample leaves the model as it was)
+    {
+        size_t need = 2;
+        for (size_t i = 0; i < P.main_slot.size(); ++i)
+            if (i < P.limits.size())
+                for (int q = 0; q < LIM_N; ++q) need += P.limits[i].block[q] >= 1;
+        if (!P.pots.empty()) {
+            need += (size_t)P.n_z_param;
+            for (const Potfile& pf : P.pots) need += (pf.ircut > 0) + (pf.isigma > 0);
+        }
+        if (need > row.size())
+            return set_error(LT_ERR_INVALID, "bayes sample " + std::to_string(k) + " has " + std::to_string(row.size()) +
+                                                 " values, the model's free parameters need " + std::to_string(need));
+    }
+    for (size_t i = 0; i < P.main_slot.size(); ++i) {
+        const size_t s = (size_t)P.main_slot[i];
+        if (i < P.limits.size()) {
+            const Limit& lim = P.limits[i];
+            if (lim.block[LIM_X] >= 1) P.x[s] = next();
+            if (lim.block[LIM_Y] >= 1) P.y[s] = next();
+            if (lim.block[LIM_EPOT] >= 1) P.epot[s] = next();
+            if (lim.block[LIM_ELL] >= 1) P.ell[s] = next();
+            if (lim.block[LIM_ANGLE] >= 1) {
+                P.angle[s] = next() * dtr;
+                P.cs[s] = std::cos(P.angle[s]);
+                P.sn[s] = std::sin(P.angle[s]);
+            }
+            if (lim.block[LIM_RCORE] >= 1) P.rcore[s] = next();
+            if (lim.block[LIM_VDISP] >= 1) P.vdisp[s] = next();
+            if (lim.block[LIM_RCUT] >= 1) P.rcut[s] = next();
+            if (lim.block[LIM_Z] >= 1) P.z[s] = next();
+        }
+        soa_parameters(P, s);
+    }
+    if (!P.pots.empty()) {
+        col += (size_t)P.n_z_param;
+        for (Potfile& pf : P.pots) {
+            if (pf.ircut > 0) pf.cut1 = next();
+            if (pf.isigma > 0) pf.sigma1 = next();
+            scale_potfile(P, pf);
+        }
     }
     return LT_OK;
 }
@@ -506,6 +674,40 @@ int lt_par_images(lt_par p, int* nimages_per_set, lt_galaxy* images)
         for (size_t k = 0; k < p->nimages.size(); ++k) nimages_per_set[k] = p->nimages[k];
     if (images)
         for (size_t k = 0; k < p->images.size(); ++k) images[k] = p->images[k];
+    return LT_OK;
+}
+
+int lt_par_bayes_load(lt_par p, const char* path, int* nsamples, int* ncolumns)
+{
+    if (!p || !path) return set_error(LT_ERR_INVALID, "NULL argument");
+    const int st = bayes_read(*p, path);
+    if (st) return st;
+    if (nsamples) *nsamples = (int)p->bayes.size();
+    if (ncolumns) *ncolumns = p->bayes_ncols;
+    return LT_OK;
+}
+
+int lt_par_bayes_apply(lt_par p, int sample)
+{
+    if (!p) return set_error(LT_ERR_NOT_FOUND, "parameter set is NULL");
+    return bayes_apply(*p, sample);
+}
+
+int lt_par_map_name(lt_par p, int kind, int which, char* buf, int cap)
+{
+    if (!p || !buf || cap <= 0) return set_error(LT_ERR_INVALID, "NULL argument");
+    int k = -1;
+    switch (kind) {
+    case LT_MAP_AMPLI: k = 0; break;
+    case LT_MAP_SHEAR: k = 1; break;
+    case LT_MAP_MASS: k = 2; break;
+    case LT_MAP_POT: k = 3; break;
+    case LT_MAP_DPL: k = which ? 5 : 4; break;
+    default: return set_error(LT_ERR_INVALID, "map kind " + std::to_string(kind) + " has no runmode file name");
+    }
+    const std::string& n = p->names[k];
+    if ((int)n.size() >= cap) return set_error(LT_ERR_INVALID, "name buffer too small for " + n);
+    std::memcpy(buf, n.c_str(), n.size() + 1);
     return LT_OK;
 }
 

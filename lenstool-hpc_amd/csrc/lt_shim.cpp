@@ -538,3 +538,44 @@ void mychi_bruteforce_SOA_CPU_grid_gradient_barycentersource(double* chi, int* e
 {
     mychi_bruteforce_SOA_GPU_grid_gradient_barycentersource(chi, error, runmode, lens, frame, nimages, images);
 }
+
+// ---- FITS ------------------------------------------------------------------------------------------
+// module_writeFits_Image, src/module_writeFits.cpp:113-200
+int module_writeFits_Image(char* filename, double* ima, int nx, int ny, double xmin, double xmax, double ymin,
+                           double ymax)
+{
+    const int st = lt_fits_write(filename, ima, nx, ny, xmin, xmax, ymin, ymax, 0, 0., 0.);
+    report(st, "module_writeFits_Image");
+    return st;
+}
+
+// module_writeFits_ImageAbsoluteCoordinates, src/module_writeFits.cpp:225-335
+int module_writeFits_ImageAbsoluteCoordinates(char* filename, double* ima, int nx, int ny, double xmin, double xmax,
+                                              double ymin, double ymax, double ra, double dec)
+{
+    const int st = lt_fits_write(filename, ima, nx, ny, xmin, xmax, ymin, ymax, 1, ra, dec);
+    report(st, "module_writeFits_ImageAbsoluteCoordinates");
+    return st;
+}
+
+// module_writeFits, src/module_writeFits.cpp:41-61 (<path>/<name>_<ii>.fits) and :63-80 (<path>/<name>.fits):
+// the absolute header unless ra == dec == 0
+void module_writeFits(std::string path, std::string filename, int ii, type_t* map, const runmode_param* runmode,
+                      int size, const grid_param* frame, type_t ra, type_t dec)
+{
+    (void)runmode;
+    const std::string file = path + "/" + filename + "_" + std::to_string(ii) + ".fits";
+    report(lt_fits_write(file.c_str(), map, size, size, frame->xmin, frame->xmax, frame->ymin, frame->ymax,
+                         !(ra == 0. && dec == 0.), ra, dec),
+           "module_writeFits");
+}
+
+void module_writeFits(std::string path, std::string filename, type_t* map, const runmode_param* runmode, int size,
+                      const grid_param* frame, type_t ra, type_t dec)
+{
+    (void)runmode;
+    const std::string file = path + "/" + filename + ".fits";
+    report(lt_fits_write(file.c_str(), map, size, size, frame->xmin, frame->xmax, frame->ymin, frame->ymax,
+                         !(ra == 0. && dec == 0.), ra, dec),
+           "module_writeFits");
+}

@@ -124,7 +124,14 @@ SIGNATURES = {
     "lt_par_get": ([C.c_void_p, C.POINTER(LtParInfo), C.POINTER(M.LtPotentialSOA), C.POINTER(M.LtGridParam),
                     C.POINTER(M.LtCosmoParam)], C.c_int),
     "lt_par_images": ([C.c_void_p, C.POINTER(C.c_int), C.POINTER(M.LtGalaxy)], C.c_int),
+    "lt_par_bayes_load": ([C.c_void_p, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
+    "lt_par_bayes_apply": ([C.c_void_p, C.c_int], C.c_int),
+    "lt_par_map_name": ([C.c_void_p, C.c_int, C.c_int, C.c_char_p, C.c_int], C.c_int),
+    "lt_map_factors": ([C.c_int, C.c_int, C.POINTER(M.LtCosmoParam), C.c_double, C.c_double, C.c_double, C.c_double,
+                        C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
     "lt_par_destroy": ([C.c_void_p], C.c_int),
+    "lt_fits_write": ([C.c_char_p, C.POINTER(C.c_double), C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                       C.c_double, C.c_int, C.c_double, C.c_double], C.c_int),
     "lt_graph_begin": ([C.c_void_p], C.c_int),
     "lt_graph_end": ([C.c_void_p, C.POINTER(C.c_void_p)], C.c_int),
     "lt_graph_launch": ([C.c_void_p, C.c_void_p], C.c_int),
@@ -459,15 +466,22 @@ class Server:
             pass
 
 
-def load_par(path):
-    """Read a Lenstool .par (and its potfiles / image file) with the library's reader. Returns a dict
-    with `info` (run settings), `lens` (LensModel, halos in the reference's SOA order), `frame`,
-    `cosmo`, `nimages` and `images` (GALAXY_DTYPE array)."""
-    h = C.c_void_p()
-    check(lib().lt_par_load(C.byref(h), str(path).encode()))
-    try:
+class ParFile:
+    """A Lenstool .par read by the library (lt_par_*): run settings, lens model, frame, cosmology,
+    images, and the bayes.dat samples of the model (`bayes_load`, `bayes_apply`, in place like the
+    reference's setbayesmapmodels)."""
+
+    def __init__(self, path):
+        h = C.c_void_p()
+        check(lib().lt_par_load(C.byref(h), str(path).encode()))
+        self.h = h
+        self.nsamples = 0
+
+    def get(self):
+        """dict with `info`, `lens` (LensModel, halos in the reference's SOA order), `extra` (vdisp,
+        ellipticity, mag, anglecos, anglesin), `frame`, `cosmo`, `nimages`, `images` (GALAXY_DTYPE)."""
         info, soa, frame, cosmo = LtParInfo(), M.LtPotentialSOA(), M.LtGridParam(), M.LtCosmoParam()
-        check(lib().lt_par_get(h, C.byref(info), C.byref(soa), C.byref(frame), C.byref(cosmo)))
+        check(lib().lt_par_get(self.h, C.byref(info), C.byref(soa), C.byref(frame), C.byref(cosmo)))
         n = info.nhalos
 
         def arr(ptr, dtype=np.float64):
@@ -479,11 +493,67 @@ def load_par(path):
                  "anglecos": arr(soa.anglecos), "anglesin": arr(soa.anglesin)}
         nimages = np.zeros(info.nsets, dtype=np.int32)
         images = M.galaxies_array(info.nimagestot)
-        check(lib().lt_par_images(h, nimages.ctypes.data_as(C.POINTER(C.c_int)), M.galaxy_ptr(images)))
+        check(lib().lt_par_images(self.h, nimages.ctypes.data_as(C.POINTER(C.c_int)), M.galaxy_ptr(images)))
         return {"info": {f: getattr(info, f) for f, _ in LtParInfo._fields_}, "lens": lens, "extra": extra,
                 "frame": frame, "cosmo": cosmo, "nimages": nimages, "images": images}
-    finally:
-        lib().lt_par_destroy(h)
+
+    def bayes_load(self, path):
+        """Read a bayes.dat; returns (samples, columns)."""
+        ns, nc = C.c_int(0), C.c_int(0)
+        check(lib().lt_par_bayes_load(self.h, str(path).encode(), C.byref(ns), C.byref(nc)))
+        self.nsamples = ns.value
+        return ns.value, nc.value
+
+    def bayes_apply(self, sample):
+        check(lib().lt_par_bayes_apply(self.h, int(sample)))
+
+    def map_name(self, kind, which=0):
+        """File name of the runmode line of a map kind ('' if absent)."""
+        buf = C.create_string_buffer(4096)
+        check(lib().lt_par_map_name(self.h, kind, which, buf, 4096))
+        return buf.value.decode()
+
+    def bayes_models(self, samples=None):
+        """LensModel of each sample, applied in order (as the reference's map loop does)."""
+        out = []
+        for k in (range(self.nsamples) if samples is None else samples):
+            self.bayes_apply(k)
+            out.append(self.get()["lens"])
+        return out
+
+    def close(self):
+        if self.h:
+            lib().lt_par_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def fits_write(path, image, xmin, xmax, ymin, ymax, ra=0.0, dec=0.0, absolute=None):
+    """Write a 2-D map (rows = y) as a float32 FITS image with the frame's coordinates; absolute
+    RA/DEC axes unless ra == dec == 0 (as the reference's module_writeFits picks)."""
+    img = np.ascontiguousarray(image, dtype=np.float64)
+    ny, nx = img.shape
+    if absolute is None:
+        absolute = not (ra == 0.0 and dec == 0.0)
+    check(lib().lt_fits_write(str(path).encode(), img.ctypes.data_as(C.POINTER(C.c_double)), nx, ny,
+                              xmin, xmax, ymin, ymax, int(bool(absolute)), ra, dec))
+
+
+def load_par(path):
+    """Read a Lenstool .par (and its potfiles / image file): ParFile(path).get()."""
+    with ParFile(path) as p:
+        return p.get()
 
 
 def stream_create() -> int:
