@@ -31,8 +31,10 @@ def hip():
 
 def _fits(path):
     raw = open(path, "rb").read()
-    cards = [raw[k:k + 80].decode() for k in range(0, 2880 * 4, 80)]
-    end = next(i for i, c in enumerate(cards) if c.startswith("END "))
+    cards = []
+    while not cards or not cards[-1].startswith("END "):
+        cards.append(raw[80 * len(cards):80 * len(cards) + 80].decode("ascii"))
+    end = len(cards) - 1
     kv = {c[:8].strip(): c[10:].split(" / ")[0].strip() for c in cards[:end] if c[8:10] == "= "}
     nx, ny = int(kv["NAXIS1"]), int(kv["NAXIS2"])
     off = ((end + 1) * 80 + 2879) // 2880 * 2880
