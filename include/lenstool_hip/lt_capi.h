@@ -263,6 +263,30 @@ int lt_par_bayes_apply(lt_par par, int sample);
 int lt_par_map_name(lt_par par, int kind, int which, char* buf, int cap);
 int lt_par_destroy(lt_par par);
 
+/* ---- critical lines and caustics ---------------------------------------------------------------------
+ * No reference counterpart: the reference parses a `cline` block (src/structure_hpc.hpp:563-578,
+ * src/module_readParameters.cpp:1008-1095, compiled out) but computes no critical line.
+ *
+ * The det A = 0 contour of a device det-A plane (grid as lt_map_device: ny rows of pitch doubles)
+ * by marching squares: per cell at most two segments (x0, y0, x1, y1), crossings linearly
+ * interpolated on the cell edges (cells sharing an edge share the point exactly), oriented with
+ * det A > 0 on the left, saddles joined on the positive side when the corner sum is positive,
+ * written to `segs` (device, 4 doubles per segment) in row-major cell order. kinds (device,
+ * optional, needs the kappa plane): 0 tangential (1 - kappa > 0 on the segment), 1 radial.
+ * *nsegments = number of segments of the whole contour; segs == NULL only counts. With more
+ * segments than max_segments the first max_segments are written and LT_ERR_OVERFLOW is returned.
+ * Waits for the stream. */
+int lt_critic_device(const double* det, const double* kappa, const lt_grid_desc* grid, double* segs, int* kinds,
+                     long long max_segments, long long* nsegments, void* stream);
+/* Critical lines of a lens for the source plane z_source on an nx x ny grid over `frame` (det A
+ * and kappa from the fields map in `precision`, then lt_critic_device), and optionally the
+ * caustics: each segment end theta mapped to beta = theta - (D_ls/D_s) alpha(theta), alpha the
+ * FP64 deflection at that point (4 doubles per segment, like `critic`). Host arrays; any of
+ * critic, caustic, kinds may be NULL. */
+int lt_critic_lines(const lt_cosmo_param* cosmo, const lt_grid_param* frame, const lt_potential_soa* lens, int nhalos,
+                    double z_lens, double z_source, int nx, int ny, int precision, double* critic, double* caustic,
+                    int* kinds, long long max_segments, long long* nsegments);
+
 /* ---- FITS maps -------------------------------------------------------------------------------------
  * Writes a host map (ny rows of nx doubles) as a FITS primary image of float32 values with the
  * frame's world coordinates: absolute == 0 gives module_writeFits_Image's linear arcsec axes

@@ -16,6 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblenstool_hip.so")
 
 LT_OK = 0
+LT_ERR_OVERFLOW = 5
 PREC_FP64 = 0
 PREC_FP32C = 1
 MAP_AMPLI, MAP_SHEAR, MAP_MASS, MAP_POT, MAP_DPL, MAP_FIELDS, MAP_HESSIAN, MAP_GRADIENT = range(1, 9)
@@ -130,6 +131,11 @@ SIGNATURES = {
     "lt_map_factors": ([C.c_int, C.c_int, C.POINTER(M.LtCosmoParam), C.c_double, C.c_double, C.c_double, C.c_double,
                         C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
     "lt_par_destroy": ([C.c_void_p], C.c_int),
+    "lt_critic_device": ([C.c_void_p, C.c_void_p, C.POINTER(LtGridDesc), C.c_void_p, C.c_void_p, C.c_longlong,
+                          C.POINTER(C.c_longlong), C.c_void_p], C.c_int),
+    "lt_critic_lines": ([C.POINTER(M.LtCosmoParam), C.POINTER(M.LtGridParam), C.POINTER(M.LtPotentialSOA), C.c_int,
+                         C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
+                         C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_longlong, C.POINTER(C.c_longlong)], C.c_int),
     "lt_fits_write": ([C.c_char_p, C.POINTER(C.c_double), C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                        C.c_double, C.c_int, C.c_double, C.c_double], C.c_int),
     "lt_graph_begin": ([C.c_void_p], C.c_int),
@@ -537,6 +543,49 @@ class ParFile:
             self.close()
         except Exception:
             pass
+
+
+def critic_device(det_ptr, kappa_ptr, grid: M.Grid, max_segments, stream=0, istart=0, jstart=0, pitch=None):
+    """Contour det A = 0 of device planes; returns (segments (n, 4), kinds or None, total, status)."""
+    g = LtGridDesc(grid.frame.xmin, grid.frame.ymin, grid.dx, grid.dy, grid.nx, grid.ny, istart, jstart,
+                   pitch or grid.nx)
+    total = C.c_longlong(0)
+    room = int(max_segments)
+    # room 0: count only (no output arrays)
+    dsegs = DeviceArray(4 * room) if room > 0 else None
+    dkinds = DeviceArray(room, dtype=np.int32) if (kappa_ptr and room > 0) else None
+    try:
+        st = lib().lt_critic_device(C.c_void_p(det_ptr), C.c_void_p(kappa_ptr) if dkinds is not None else None,
+                                    C.byref(g), C.c_void_p(dsegs.ptr) if dsegs is not None else None,
+                                    C.c_void_p(dkinds.ptr) if dkinds is not None else None,
+                                    room, C.byref(total), C.c_void_p(stream))
+        if st not in (LT_OK, LT_ERR_OVERFLOW):
+            check(st)
+        got = min(total.value, room)
+        segs = dsegs.download(4 * got).reshape(got, 4) if got > 0 else np.zeros((0, 4))
+        kinds = (dkinds.download(got) if got > 0 else np.zeros(0, np.int32)) if dkinds is not None else None
+        return segs, kinds, total.value, st
+    finally:
+        if dsegs is not None:
+            dsegs.close()
+        if dkinds is not None:
+            dkinds.close()
+
+
+def critic_lines(lens: M.LensModel, frame, nx, ny, cosmo, z_lens, z_source, precision=PREC_FP64,
+                 max_segments=1 << 20, caustics=True):
+    """Critical lines (n, 4), caustics (n, 4) or None, kinds (n,) of a lens; total segment count."""
+    crit = np.zeros((max_segments, 4))
+    caus = np.zeros((max_segments, 4)) if caustics else None
+    kinds = np.zeros(max_segments, dtype=np.int32)
+    total = C.c_longlong(0)
+    st = lib().lt_critic_lines(C.byref(cosmo), C.byref(frame), C.byref(lens.soa()), lens.n, z_lens, z_source, nx, ny,
+                               precision, M.dptr(crit), M.dptr(caus) if caus is not None else None,
+                               kinds.ctypes.data_as(C.POINTER(C.c_int)), max_segments, C.byref(total))
+    if st not in (LT_OK, LT_ERR_OVERFLOW):
+        check(st)
+    got = min(total.value, max_segments)
+    return crit[:got], (caus[:got] if caus is not None else None), kinds[:got], total.value
 
 
 def fits_write(path, image, xmin, xmax, ymin, ymax, ra=0.0, dec=0.0, absolute=None):

@@ -778,3 +778,101 @@ int orc_chi_barycentersource(double* chi, int* error, int n, int nsets, int nhal
     free(loc);
     return st;
 }
+
+/* ---- critical lines (marching squares; see lt_oracle.h) ----------------------------------------- */
+
+static double orc_coord(double min, int i, double d)
+{
+    double t = (double)i * d;
+    return min + t;
+}
+
+/* crossing on edge e (0 bottom, 1 right, 2 top, 3 left) of cell (gi, gj), corners c[4] */
+static void orc_edge_point(int e, const double* c, int gi, int gj, double xmin, double ymin, double dx, double dy,
+                           double* x, double* y, double* t)
+{
+    if (e == 0) {
+        *t = c[0] / (c[0] - c[1]);
+        *x = orc_coord(xmin, gi, dx) + *t * dx;
+        *y = orc_coord(ymin, gj, dy);
+    } else if (e == 1) {
+        *t = c[1] / (c[1] - c[2]);
+        *x = orc_coord(xmin, gi + 1, dx);
+        *y = orc_coord(ymin, gj, dy) + *t * dy;
+    } else if (e == 2) {
+        *t = c[3] / (c[3] - c[2]);
+        *x = orc_coord(xmin, gi, dx) + *t * dx;
+        *y = orc_coord(ymin, gj + 1, dy);
+    } else {
+        *t = c[0] / (c[0] - c[3]);
+        *x = orc_coord(xmin, gi, dx);
+        *y = orc_coord(ymin, gj, dy) + *t * dy;
+    }
+}
+
+static double orc_edge_value(int e, double t, const double* k)
+{
+    static const int a[4] = {0, 1, 3, 0}, b[4] = {1, 2, 2, 3};
+    return k[a[e]] + t * (k[b[e]] - k[a[e]]);
+}
+
+/* edge pairs per corner mask: each entry lists up to two (from, to) segments; saddles resolved below */
+static int orc_cell_table(const double* c, int* e)
+{
+    int m = (c[0] > 0.) | ((c[1] > 0.) << 1) | ((c[2] > 0.) << 2) | ((c[3] > 0.) << 3);
+    /* single segments: from the edge where the positive side begins, positive on the left */
+    static const int from[16] = {-1, 0, 1, 1, 2, -1, 2, 2, 3, 0, -1, 1, 3, 0, 3, -1};
+    static const int to[16] = {-1, 3, 0, 3, 1, -1, 0, 3, 2, 2, -1, 2, 1, 1, 0, -1};
+    double sum = (c[0] + c[1]) + (c[2] + c[3]);
+    if (m == 5) {
+        if (sum > 0.) { e[0] = 0; e[1] = 1; e[2] = 2; e[3] = 3; }
+        else { e[0] = 0; e[1] = 3; e[2] = 2; e[3] = 1; }
+        return 2;
+    }
+    if (m == 10) {
+        if (sum > 0.) { e[0] = 3; e[1] = 0; e[2] = 1; e[3] = 2; }
+        else { e[0] = 1; e[1] = 0; e[2] = 3; e[3] = 2; }
+        return 2;
+    }
+    if (from[m] < 0) return 0;
+    e[0] = from[m];
+    e[1] = to[m];
+    return 1;
+}
+
+int orc_critic_lines(double* segs, int* kinds, long max_seg, long* nseg, const double* det, const double* kappa,
+                     int nx, int ny, long pitch, double xmin, double ymin, double dx, double dy, int istart,
+                     int jstart)
+{
+    if (!det || !nseg || nx < 2 || ny < 2 || pitch < nx) return LT_ERR_INVALID;
+    if (kinds && !kappa) return LT_ERR_INVALID;
+    long n = 0;
+    for (int j = 0; j < ny - 1; ++j) {
+        for (int i = 0; i < nx - 1; ++i) {
+            const double* r0 = det + (long)j * pitch;
+            const double* r1 = r0 + pitch;
+            double c[4] = {r0[i], r0[i + 1], r1[i + 1], r1[i]};
+            int e[4];
+            int ns = orc_cell_table(c, e);
+            for (int q = 0; q < ns; ++q, ++n) {
+                if (!segs || n >= max_seg) continue;
+                double xa, ya, ta, xb, yb, tb;
+                orc_edge_point(e[2 * q], c, istart + i, jstart + j, xmin, ymin, dx, dy, &xa, &ya, &ta);
+                orc_edge_point(e[2 * q + 1], c, istart + i, jstart + j, xmin, ymin, dx, dy, &xb, &yb, &tb);
+                segs[4 * n] = xa;
+                segs[4 * n + 1] = ya;
+                segs[4 * n + 2] = xb;
+                segs[4 * n + 3] = yb;
+                if (kinds) {
+                    const double* k0 = kappa + (long)j * pitch;
+                    const double* k1 = k0 + pitch;
+                    double k[4] = {k0[i], k0[i + 1], k1[i + 1], k1[i]};
+                    double ka = orc_edge_value(e[2 * q], ta, k), kb = orc_edge_value(e[2 * q + 1], tb, k);
+                    kinds[n] = 1. - 0.5 * (ka + kb) > 0. ? 0 : 1;
+                }
+            }
+        }
+    }
+    *nseg = n;
+    return (segs && n > max_seg) ? LT_ERR_OVERFLOW : LT_OK;
+}

@@ -106,6 +106,19 @@ int orc_chi_barycentersource(double* chi, int* error, int nbgridcells, int nsets
                              const lt_potential_soa* lens, const lt_grid_param* frame,
                              const int* nimages, const lt_galaxy* images);
 
+/* ---- critical lines -------------------------------------------------------------------------- */
+/* det A = 0 contour of a det-A plane (ny rows of `pitch` values, pixel (i, j) at
+ * (xmin + (istart+i) dx, ymin + (jstart+j) dy)) by marching squares: the library's rules (no
+ * reference counterpart; the reference has no critical-line code): per cell, corner bit k set when
+ * c_k > 0 (c0 = (i,j), c1 = (i+1,j), c2 = (i+1,j+1), c3 = (i,j+1)); crossings at t = va/(va - vb)
+ * from the first end of an edge (bottom/top from the left, left/right from below); segments
+ * oriented with positive det on the left; saddles joined on the positive side when c0+c1+c2+c3 > 0;
+ * row-major cell order. kinds (if kappa given): 0 when 1 - mean kappa of the ends > 0, else 1.
+ * Writes min(total, max_seg) segments (x0, y0, x1, y1), *nseg = total. Single-threaded. */
+int orc_critic_lines(double* segs, int* kinds, long max_seg, long* nseg, const double* det, const double* kappa,
+                     int nx, int ny, long pitch, double xmin, double ymin, double dx, double dy, int istart,
+                     int jstart);
+
 /* number of OpenMP threads the grid drivers use (0 = library default) */
 void orc_set_threads(int n);
 int  orc_get_threads(void);

@@ -74,6 +74,8 @@ def lib():
         L.orc_cond_planes.argtypes = [dp, dp, gp, soa, C.c_int, C.c_double, C.c_double, C.c_int,
                                       C.c_int, C.c_int, C.c_int]
         L.orc_point_cond.argtypes = [dp, dp, P(M.LtPoint), soa, C.c_int]
+        L.orc_critic_lines.argtypes = [dp, P(C.c_int), C.c_long, P(C.c_long), dp, dp, C.c_int, C.c_int, C.c_long,
+                                       C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int]
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_get_threads.restype = C.c_int
         _lib = L
@@ -279,3 +281,23 @@ def chi_barycentersource(lens: M.LensModel, n, frame, nimages, images):
                                         C.byref(lens.soa()), C.byref(frame),
                                         nimages.ctypes.data_as(C.POINTER(C.c_int)), M.galaxy_ptr(images))
     return st, chi.value
+
+
+def critic_lines(det, kappa, grid: M.Grid, istart=0, jstart=0, max_seg=None):
+    """(segments (n, 4), kinds (n,) or None) of the det = 0 contour of a (ny, nx) det plane."""
+    det = np.ascontiguousarray(det, dtype=np.float64)
+    ny, nx = det.shape
+    kap = None if kappa is None else np.ascontiguousarray(kappa, dtype=np.float64)
+    n = C.c_long(0)
+    _chk(lib().orc_critic_lines(None, None, 0, C.byref(n), M.dptr(det), M.dptr(kap) if kap is not None else None,
+                                nx, ny, nx, grid.frame.xmin, grid.frame.ymin, grid.dx, grid.dy, istart, jstart),
+         "orc_critic_lines")
+    total = n.value if max_seg is None else min(n.value, max_seg)
+    segs = np.zeros((max(total, 1), 4))
+    kinds = np.zeros(max(total, 1), dtype=np.int32) if kap is not None else None
+    st = lib().orc_critic_lines(M.dptr(segs), kinds.ctypes.data_as(C.POINTER(C.c_int)) if kinds is not None else None,
+                                total, C.byref(n), M.dptr(det), M.dptr(kap) if kap is not None else None, nx, ny, nx,
+                                grid.frame.xmin, grid.frame.ymin, grid.dx, grid.dy, istart, jstart)
+    if st not in (0, 5):
+        _chk(st, "orc_critic_lines")
+    return segs[:total], (kinds[:total] if kinds is not None else None)

@@ -1,0 +1,107 @@
+"""Critical lines (marching squares on the GPU's det A plane) and caustics.
+
+The device contour must equal the oracle's contour of the same planes bit for bit (segments, kinds,
+count, order), on the real A2744 model with a pitched plane; count-only and overflow calls must
+agree with it. lt_critic_lines is pinned by closed forms: for a singular isothermal sphere the
+critical line is the circle r = b0 D_ls/D_s and the caustic its centre (both within the linear
+interpolation's O(h^2)); a cored dPIE has an inner radial and an outer tangential line."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import ltmodel as M
+import pyoracle as O
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def hip():
+    import lthip
+    lthip.check(lthip.lib().lt_init(0))
+    return lthip
+
+
+def _fields_device(hip, lens, grid, r, pitch, precision=0):
+    m = hip.Model(lens, hip.MAP_HESSIAN)
+    kap = hip.DeviceArray(pitch * grid.ny)
+    det = hip.DeviceArray(pitch * grid.ny)
+    kap.fill_bytes(0)
+    det.fill_bytes(0)
+    m.map(hip.MAP_FIELDS, 0, [kap.ptr, 0, 0, det.ptr, 0], grid, r=r, precision=precision, pitch=pitch)
+    hip.sync()
+    m.close()
+    return kap, det
+
+
+def test_device_contour_equals_oracle(hip):
+    d = hip.load_par(os.path.join(GOLDEN, "par", "A2744_SLspec.par"))
+    lens = d["lens"]
+    grid = M.Grid(M.grid_param(-70.0, 70.0, -60.0, 50.0), 300, 240)
+    r = hip.cosmo_dlsds(0.308, 2.0, d["cosmo"])
+    pitch = grid.nx + 7
+    print(f"[case] A2744 critical lines: {lens.n} halos, {grid.nx}x{grid.ny}, pitch {pitch}", flush=True)
+    kap, det = _fields_device(hip, lens, grid, r, pitch)
+    try:
+        hk = kap.download().reshape(grid.ny, pitch)[:, :grid.nx]
+        hd = det.download().reshape(grid.ny, pitch)[:, :grid.nx]
+        want, wkinds = O.critic_lines(hd, hk, grid)
+        assert len(want) > 200 and (wkinds == 0).any() and (wkinds == 1).any()
+        got, kinds, total, st = hip.critic_device(det.ptr, kap.ptr, grid, 1 << 16, pitch=pitch)
+        assert st == hip.LT_OK and total == len(want)
+        np.testing.assert_array_equal(got, want)
+        np.testing.assert_array_equal(kinds, wkinds)
+        # count only; then room for 10: the first 10 and LT_ERR_OVERFLOW
+        _, _, total0, st0 = hip.critic_device(det.ptr, 0, grid, 0, pitch=pitch)
+        assert st0 == hip.LT_OK and total0 == total
+        part, pk, total10, st10 = hip.critic_device(det.ptr, kap.ptr, grid, 10, pitch=pitch)
+        assert st10 == hip.LT_ERR_OVERFLOW and total10 == total
+        np.testing.assert_array_equal(part, want[:10])
+        np.testing.assert_array_equal(pk, wkinds[:10])
+    finally:
+        kap.close()
+        det.close()
+
+
+def test_sis_circle_and_point_caustic(hip):
+    cosmo = M.flat_lcdm()
+    zl, zs = 0.3, 1.5
+    lens = M.LensModel.from_halos([{"type": 5, "x": 0.0, "y": 0.0, "sigma": 900.0, "e": 0.0}], zl=zl)
+    r = hip.cosmo_dlsds(zl, zs, cosmo)
+    theta_e = lens.b0[0] * r
+    n, half = 256, 2.5 * theta_e
+    frame = M.grid_param(-half, half, -half, half)
+    h = 2 * half / (n - 1)
+    print(f"[case] SIS critical circle theta_E={theta_e:.4f} n={n}", flush=True)
+    crit, caus, kinds, total = hip.critic_lines(lens, frame, n, n, cosmo, zl, zs)
+    assert total == len(crit) > 100
+    for px, py in ((crit[:, 0], crit[:, 1]), (crit[:, 2], crit[:, 3])):
+        assert np.abs(np.hypot(px, py) - theta_e).max() < h * h / theta_e
+    assert np.hypot(caus[:, 0], caus[:, 1]).max() < h * h / theta_e + 1e-12
+    assert (kinds == 0).all()
+    # closed and clockwise
+    assert sorted(map(tuple, crit[:, :2])) == sorted(map(tuple, crit[:, 2:]))
+    area = 0.5 * np.sum(crit[:, 0] * crit[:, 3] - crit[:, 2] * crit[:, 1])
+    assert area < 0 and abs(-area / (math.pi * theta_e ** 2) - 1) < 1e-3
+
+
+def test_cored_dpie_radial_and_tangential(hip):
+    cosmo = M.flat_lcdm()
+    zl, zs = 0.3, 2.0
+    lens = M.LensModel.from_halos([{"type": 81, "x": 0.0, "y": 0.0, "sigma": 1100.0, "e": 0.2, "theta": 0.4,
+                                    "rcore": 2.0, "rcut": 300.0}], zl=zl)
+    frame = M.grid_param(-60.0, 60.0, -60.0, 60.0)
+    print("[case] cored dPIE critical lines 400x400", flush=True)
+    for prec in (0, 1):
+        crit, caus, kinds, total = hip.critic_lines(lens, frame, 400, 400, cosmo, zl, zs, precision=prec)
+        assert total == len(crit) > 100
+        rad = np.hypot(crit[kinds == 1, 0], crit[kinds == 1, 1])
+        tan = np.hypot(crit[kinds == 0, 0], crit[kinds == 0, 1])
+        assert len(rad) > 10 and len(tan) > 10 and rad.max() < tan.min()
+        # the tangential line maps to a small diamond-like caustic inside the radial line's caustic
+        ct = np.hypot(caus[kinds == 0, 0], caus[kinds == 0, 1])
+        cr = np.hypot(caus[kinds == 1, 0], caus[kinds == 1, 1])
+        assert np.median(ct) < np.median(cr)

@@ -17,7 +17,7 @@ HIPCC = "/opt/rocm/bin/hipcc"
 
 
 @pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not available")
-@pytest.mark.parametrize("src", ["lt_kernels64.hip", "lt_kernels32.hip"])
+@pytest.mark.parametrize("src", ["lt_kernels64.hip", "lt_kernels32.hip", "lt_critic.hip"])
 def test_no_undefined_scalar_registers_in_kernels(src):
     path = os.path.join(CSRC, src)
     if not os.path.exists(path):
