@@ -261,6 +261,19 @@ int lt_par_bayes_apply(lt_par par, int sample);
 /* File name of a runmode map line (`ampli`, `shear`, `mass`, `poten`; `dpl` has two: which = 0 for
  * x, 1 for y), as read by read_runmode (src/module_readParameters.cpp:415-465); "" if absent. */
 int lt_par_map_name(lt_par par, int kind, int which, char* buf, int cap);
+/* The `cline` block (cline_param, src/structure_hpc.hpp:563-578; its reader
+ * src/module_readParameters.cpp:1008-1095 is compiled out in the reference): source planes
+ * (nplan clamped to LT_CLINE_MAXPLANES), the half-width dmax of the square frame, the step
+ * (`pas` / `step` / `limitLow`), limitHigh and nbgridcells (0 if absent). Defaults as the
+ * reference's reader: dmax 1, step 1, limitHigh 10. */
+#define LT_CLINE_MAXPLANES 9
+typedef struct lt_cline_info {
+    int nplan;
+    double z[LT_CLINE_MAXPLANES];
+    double dmax, step, limit_high;
+    int nbgridcells;
+} lt_cline_info;
+int lt_par_cline(lt_par par, lt_cline_info* cline);
 int lt_par_destroy(lt_par par);
 
 /* ---- critical lines and caustics ---------------------------------------------------------------------

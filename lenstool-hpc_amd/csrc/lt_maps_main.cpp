@@ -13,7 +13,10 @@
 // every sample's displacement map to the same file, only its last sample survives there).
 // Map planes: the runmode line's cells x cells over the frame, z from the runmode line; the mass
 // map's lens plane is the runmode's z (the reference passes runmode.z_mass), the source plane the
-// source block's z_source; other maps use the first halo's z as lens plane.
+// source block's z_source; other maps use the first halo's z as lens plane. With a `cline` block of
+// nplan >= 1 source planes, the critical lines and caustics of every sample for every plane go to
+// <DIR>/critic_<plane>[_<k>].dat and caustic_<plane>[_<k>].dat, one segment per line
+// (x0 y0 x1 y1 kind; kind 0 tangential, 1 radial), on the square frame of half-width dmax.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -158,8 +161,10 @@ int main(int argc, char** argv)
     add(info.shear, LT_MAP_SHEAR, info.shear_cells, info.shear_z);
     add(info.dpl, LT_MAP_DPL, info.dpl_cells, info.dpl_z);
     add(info.poten, LT_MAP_POT, info.poten_cells, info.poten_z);
-    if (jobs.empty()) {
-        std::fprintf(stderr, "lenstool_maps: the runmode block asks for no map\n");
+    lt_cline_info cline{};
+    if (fail(lt_par_cline(par, &cline), "lt_par_cline")) return 1;
+    if (jobs.empty() && cline.nplan <= 0) {
+        std::fprintf(stderr, "lenstool_maps: the parameter file asks for no map and no critical line\n");
         return 1;
     }
 
@@ -254,11 +259,57 @@ int main(int argc, char** argv)
         if (model) lt_model_destroy(model);
         for (double* p : dev) lt_dev_free(p);
     }
+    // critical lines and caustics of every sample for every `cline` source plane, on the square
+    // frame of half-width dmax with nbgridcells (or 2 dmax / step + 1) pixels a side
+    int nlines = 0;
+    if (cline.nplan > 0) {
+        const int nc = cline.nbgridcells > 1 ? cline.nbgridcells
+                                             : std::min(16385, std::max(2, (int)(2 * cline.dmax / cline.step) + 1));
+        lt_grid_param cf = frame;
+        cf.xmin = cf.ymin = -cline.dmax;
+        cf.xmax = cf.ymax = cline.dmax;
+        long long room = 1 << 16;
+        std::vector<double> crit, caus;
+        std::vector<int> kinds;
+        for (int p = 0; p < cline.nplan; ++p)
+            for (int k = 0; k < nsamples; ++k) {
+                const Soa& m = models[(size_t)k];
+                long long nseg = 0;
+                int st = LT_ERR_OVERFLOW;
+                while (st == LT_ERR_OVERFLOW) {
+                    crit.resize((size_t)room * 4);
+                    caus.resize((size_t)room * 4);
+                    kinds.resize((size_t)room);
+                    st = lt_critic_lines(&cosmo, &cf, &m.soa, nh, m.d[9][0], cline.z[p], nc, nc, o.precision,
+                                         crit.data(), caus.data(), kinds.data(), room, &nseg);
+                    if (st == LT_ERR_OVERFLOW) room = nseg;
+                }
+                if (fail(st, "lt_critic_lines")) return 1;
+                const std::string tag = std::to_string(p) + (bayes ? "_" + std::to_string(k) : std::string());
+                for (int w = 0; w < 2; ++w) {
+                    const std::string path = o.out + (w ? "/caustic_" : "/critic_") + tag + ".dat";
+                    std::FILE* f = std::fopen(path.c_str(), "w");
+                    if (!f) {
+                        std::fprintf(stderr, "lenstool_maps: cannot write %s\n", path.c_str());
+                        return 1;
+                    }
+                    std::fprintf(f, "# %s of source plane z = %.6f: x0 y0 x1 y1 [arcsec] kind (0 tangential, 1 radial)\n",
+                                 w ? "caustic segments" : "critical-line segments", cline.z[p]);
+                    const double* v = w ? caus.data() : crit.data();
+                    for (long long q = 0; q < nseg; ++q)
+                        std::fprintf(f, "%.10f %.10f %.10f %.10f %d\n", v[4 * q], v[4 * q + 1], v[4 * q + 2],
+                                     v[4 * q + 3], kinds[(size_t)q]);
+                    std::fclose(f);
+                    if (!o.quiet) std::printf("WRITE: %s [%lld segments]\n", path.c_str(), nseg);
+                }
+                ++nlines;
+            }
+    }
     t_all += now_s();
     lt_stream_destroy(stream);
     lt_par_destroy(par);
     std::printf("maps: %d sample(s) x %zu map kind(s), %d halos; GPU %.3f s (models, maps, copies), total %.3f s, "
-                "%.3e halo*pixel/s on the GPU\n",
-                nsamples, jobs.size(), nh, t_gpu, t_all, t_gpu > 0 ? (double)pixels * nh / t_gpu : 0.0);
+                "%.3e halo*pixel/s on the GPU; %d critical-line set(s)\n",
+                nsamples, jobs.size(), nh, t_gpu, t_all, t_gpu > 0 ? (double)pixels * nh / t_gpu : 0.0, nlines);
     return 0;
 }

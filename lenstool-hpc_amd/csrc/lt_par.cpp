@@ -18,6 +18,7 @@
 // positions go through the relative -> absolute -> relative coordinate round trip; `profil` inside a
 // potfile block is ignored (its `type` line sets the profile). Unlike the reference, a missing
 // catalogue or redshift is an error status instead of exit(-1).
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -150,6 +151,7 @@ struct lt_par_s {
     std::vector<std::vector<double>> bayes;
     // runmode map file names: ampli, shear, mass, poten, dpl x, dpl y
     std::string names[6];
+    lt_cline_info cline{};
 
     void push(const Halo& h)
     {
@@ -241,6 +243,10 @@ int parse(lt_par_s& P, const std::string& path)
     C.wa = 0.;
     C.curvature = 0.;
     lt_par_info& R = P.info;
+    P.cline.dmax = 1;
+    P.cline.step = 1;
+    P.cline.limit_high = 10;
+    P.cline.nbgridcells = 0;
     R.nbgridcells = 1000;
     R.ampli_cells = R.mass_cells = R.shear_cells = R.poten_cells = R.dpl_cells = 1000;
     R.mass_z = 0.4;
@@ -343,6 +349,25 @@ int parse(lt_par_s& P, const std::string& path)
                 else if (k == "xmax") F.xmax = v;
                 else if (k == "ymin") F.ymin = v;
                 else if (k == "ymax") F.ymax = v;
+            }
+        } else if (starts(first, "cline")) {
+            // critical-line planes (the reference's compiled-out reader,
+            // src/module_readParameters.cpp:1008-1095): nplan n z1 z2 ..., dmax, pas / step /
+            // limitLow, limitHigh, nbgridcells
+            lt_cline_info& K = P.cline;
+            for (++i; i < L.size() && !block_end(i); ++i) {
+                const std::string k = tok(L[i], 0);
+                if (k == "nplan") {
+                    K.nplan = std::min(std::max(std::atoi(tok(L[i], 1).c_str()), 0), LT_CLINE_MAXPLANES);
+                    for (int q = 0; q < LT_CLINE_MAXPLANES; ++q) {
+                        const std::string w = tok(L[i], 2 + q);
+                        if (w.empty()) break;
+                        K.z[q] = std::atof(w.c_str());
+                    }
+                } else if (k == "dmax") K.dmax = std::atof(tok(L[i], 1).c_str());
+                else if (k == "pas" || k == "step" || k == "limitLow") K.step = std::atof(tok(L[i], 1).c_str());
+                else if (k == "limitHigh") K.limit_high = std::atof(tok(L[i], 1).c_str());
+                else if (k == "nbgridcells") K.nbgridcells = std::atoi(tok(L[i], 1).c_str());
             }
         } else if (starts(first, "potfile")) {
             Potfile pf;
@@ -691,6 +716,13 @@ int lt_par_bayes_apply(lt_par p, int sample)
 {
     if (!p) return set_error(LT_ERR_NOT_FOUND, "parameter set is NULL");
     return bayes_apply(*p, sample);
+}
+
+int lt_par_cline(lt_par p, lt_cline_info* cline)
+{
+    if (!p || !cline) return set_error(LT_ERR_INVALID, "NULL argument");
+    *cline = p->cline;
+    return LT_OK;
 }
 
 int lt_par_map_name(lt_par p, int kind, int which, char* buf, int cap)

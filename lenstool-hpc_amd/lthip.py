@@ -128,6 +128,7 @@ SIGNATURES = {
     "lt_par_bayes_load": ([C.c_void_p, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
     "lt_par_bayes_apply": ([C.c_void_p, C.c_int], C.c_int),
     "lt_par_map_name": ([C.c_void_p, C.c_int, C.c_int, C.c_char_p, C.c_int], C.c_int),
+    "lt_par_cline": ([C.c_void_p, C.c_void_p], C.c_int),
     "lt_map_factors": ([C.c_int, C.c_int, C.POINTER(M.LtCosmoParam), C.c_double, C.c_double, C.c_double, C.c_double,
                         C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
     "lt_par_destroy": ([C.c_void_p], C.c_int),
@@ -472,6 +473,11 @@ class Server:
             pass
 
 
+class LtClineInfo(C.Structure):
+    _fields_ = [("nplan", C.c_int), ("z", C.c_double * 9), ("dmax", C.c_double), ("step", C.c_double),
+                ("limit_high", C.c_double), ("nbgridcells", C.c_int)]
+
+
 class ParFile:
     """A Lenstool .par read by the library (lt_par_*): run settings, lens model, frame, cosmology,
     images, and the bayes.dat samples of the model (`bayes_load`, `bayes_apply`, in place like the
@@ -518,6 +524,13 @@ class ParFile:
         buf = C.create_string_buffer(4096)
         check(lib().lt_par_map_name(self.h, kind, which, buf, 4096))
         return buf.value.decode()
+
+    def cline(self):
+        """The `cline` block: nplan, z (list), dmax, step, limit_high, nbgridcells."""
+        k = LtClineInfo()
+        check(lib().lt_par_cline(self.h, C.byref(k)))
+        return {"nplan": k.nplan, "z": list(k.z), "dmax": k.dmax, "step": k.step, "limit_high": k.limit_high,
+                "nbgridcells": k.nbgridcells}
 
     def bayes_models(self, samples=None):
         """LensModel of each sample, applied in order (as the reference's map loop does)."""

@@ -7,6 +7,7 @@ critical line is the circle r = b0 D_ls/D_s and the caustic its centre (both wit
 interpolation's O(h^2)); a cored dPIE has an inner radial and an outer tangential line."""
 import math
 import os
+import subprocess
 
 import numpy as np
 import pytest
@@ -105,3 +106,56 @@ def test_cored_dpie_radial_and_tangential(hip):
         ct = np.hypot(caus[kinds == 0, 0], caus[kinds == 0, 1])
         cr = np.hypot(caus[kinds == 1, 0], caus[kinds == 1, 1])
         assert np.median(ct) < np.median(cr)
+
+
+SIS_PAR = """runmode
+\tampli 1 101 1.5 ampli_sis
+\tend
+source
+\tz_source 1.5
+\tend
+cosmologie
+\tH0 70.
+\tomega 0.3
+\tlambda 0.7
+\tend
+champ
+\tdmax 30.
+\tend
+potentiel S
+\tprofil 5
+\tx_centre 0.
+\ty_centre 0.
+\tellipticite 0.
+\tv_disp 700.
+\tz_lens 0.3
+\tend
+cline
+\tnplan 1 1.5
+\tdmax 20.
+\tnbgridcells 201
+\tend
+fini
+"""
+
+
+def test_maps_tool_critical_lines(hip, tmp_path):
+    """lenstool_maps with a cline block: critic_0.dat holds the SIS critical circle r = b0 D_ls/D_s
+    and caustic_0.dat points at the centre, both within the O(h^2) of the interpolation."""
+    tool = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "lenstool-hpc_amd", "lenstool_maps")
+    par = tmp_path / "sis.par"
+    par.write_text(SIS_PAR)
+    print("[case] lenstool_maps cline: SIS, 201x201 over +-20", flush=True)
+    r = subprocess.run([tool, str(par), "--out", str(tmp_path), "--quiet"], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-1000:], r.stderr[-1000:], flush=True)
+    assert r.returncode == 0
+    crit = np.loadtxt(tmp_path / "critic_0.dat")
+    caus = np.loadtxt(tmp_path / "caustic_0.dat")
+    d = hip.load_par(str(par))
+    theta_e = d["lens"].b0[0] * hip.cosmo_dlsds(0.3, 1.5, d["cosmo"])
+    h = 40.0 / 200
+    assert len(crit) > 50 and len(caus) == len(crit) and (crit[:, 4] == 0).all()
+    # 10 decimals in the file
+    assert np.abs(np.hypot(crit[:, 0], crit[:, 1]) - theta_e).max() < h * h / theta_e + 1e-9
+    assert np.hypot(caus[:, 0], caus[:, 1]).max() < h * h / theta_e + 1e-9
+    assert os.path.exists(tmp_path / "ampli_sis.fits")

@@ -257,3 +257,12 @@ def test_catalogue_beside_the_par(lt, tmp_path):
         shutil.copyfile(os.path.join(MAPS, f), tmp_path / f)
     d = lt.load_par(str(tmp_path / "m1931.par"))
     assert d["info"]["nhalos"] == 209
+
+
+def test_cline_block(lt):
+    """utils/maps' m1931.par: `nplan 0 1.49 1.894 2.497`, `dmax 150.0`, `pas .1` -- no plane asked,
+    the redshifts still read, as the reference's reader reads them."""
+    with lt.ParFile(os.path.join(MAPS, "m1931.par")) as p:
+        k = p.cline()
+    assert k["nplan"] == 0 and k["z"][:3] == [1.49, 1.894, 2.497] and k["dmax"] == 150.0 and k["step"] == 0.1
+    assert k["limit_high"] == 10 and k["nbgridcells"] == 0
