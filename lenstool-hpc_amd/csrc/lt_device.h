@@ -11,11 +11,12 @@
 
 namespace lt {
 
-// One halo, 40 doubles. Field meaning by type:
+// One halo, 48 doubles. Field meaning by type:
 //   all     : xh, yh (centre), cs, sn (cos/sin of the angle), hc2, hs2 (Hessian amplitude times
 //             cos/sin of twice the angle), b0, eps, rc, rcut (raw parameters, for the potential)
-//   8, 81   : PIEMD constants (see below); ampg / amph gradient / Hessian amplitudes; e2ab .. qs2h
-//             the constants of the division-free Hessian form
+//   8, 81   : PIEMD constants (see below); ampg / amph gradient / Hessian amplitudes; csx .. k5,
+//             ra2c, rb2c, qT .. qs2h the constants of the Hessian's scaled division-free form
+//             (lt_terms64.h)
 //   5       : sa = 1 - eps, sb = 1 + eps, ampg = b0, amph = b0 (1 - eps)(1 + eps)
 //   14      : h14a, h14b, h14c = constant Hessian (lab frame)
 struct alignas(64) HaloRec {
@@ -28,14 +29,17 @@ struct alignas(64) HaloRec {
     double e2ra, e2rb;              // 14, 15 2 rcore sqrt(e), 2 rcut sqrt(e)
     double ampg;                    // 16     gradient amplitude
     double amph;                    // 17     Hessian amplitude
-    double hk1, hk2, hk3, hk4, hk5, hk6;  // 18..23 Hessian bracket coefficients
+    double csx, snx, csy, sny;      // 18..21 cx1 a (cs, sn), cx1 b (cs, sn): the scaled Hessian frame
+    double ra2c, rb2c;              // 22, 23 cx1^2 rcore^2, cx1^2 rcut^2
     double sa, sb;                  // 24, 25 SIS (1 - e), (1 + e)
     double h14a, h14b, h14c;        // 26..28 type-14 constant Hessian
     double b0, eps, rc, rcut;       // 29..32 raw parameters (potential)
-    double e2ab, e2a2, cxa2;        // 33..35 2 sqrt(e) a b, 2 sqrt(e) a^2, cx1 a^2 (a, b = 1/(1 +- e))
-    double qT, qc2, qs2m2, qs2h;    // 36..39 Hessian amplitudes of the division-free form (lt_terms64.h)
+    double k1, k2, k3;              // 33..35 2 sqrt(e) a / cx1, b / a, 2 sqrt(e) b / cx1
+    double qT, qc2, qs2m2, qs2h;    // 36..39 Hessian amplitudes of the division-free form
+    double k4, k5;                  // 40, 41 1 / cx1^2, 2 sqrt(e) a   (a, b = 1/(1 +- e))
+    double pad[6];                  // 42..47
 };
-static_assert(sizeof(HaloRec) == 320, "halo record is 320 bytes");
+static_assert(sizeof(HaloRec) == 384, "halo record is 384 bytes");
 
 // A run of halos of one type (and, for the Kmap, one dlsds weight).
 struct alignas(32) Run {
@@ -50,22 +54,18 @@ static_assert(sizeof(Run) == 32, "run is 32 bytes");
 
 // Single-precision record of a halo for the FP32-compensated Hessian. The PIEMD terms use a form
 // of the I_{0.5} Hessian without the removable 0/0 of the closed form (see lt_kernels32.hip), whose
-// constants are these. The centre is split into hi + lo floats so that pixel - centre keeps FP64
-// accuracy in FP32 arithmetic. near2: squared halo-frame radius inside which the FP32 rounding of
+// constants are these. The offset pixel - centre is formed from the FP64 centre of the HaloRec and
+// rounded once to FP32. near2: squared halo-frame radius inside which the FP32 rounding of
 // this halo's term would exceed the error budget; there the FP64 term of the HaloRec is used.
 struct alignas(64) HaloRec32 {
-    float xh_hi, xh_lo, yh_hi, yh_lo;   // 0..3
-    float cs, sn;                       // 4, 5
-    float a2, b2;                       // 6, 7    a^2 = 1/(1+e)^2, b^2 = 1/(1-e)^2
-    float e2, cx1, e2ab, e2a2;          // 8..11   2 sqrt(e), (1-e)/(1+e), 2 sqrt(e) a b, 2 sqrt(e) a^2
-    float cxa2;                         // 12      cx1 a^2
-    float ra, ra2;                      // 13, 14  rcore, rcore^2
-    float rb, rb2;                      // 15, 16  rcut, rcut^2
-    float icx1, pad0;                   // 17, 18  1 / cx1
-    float hT, hc2, hs2m2, hs2h;         // 19..22  lab-frame amplitudes: trace, cos 2t, -2 sin 2t, sin 2t / 2
-    float near2;                        // 23
-    float sa, sb;                       // 24, 25  SIS: 1 - e, 1 + e
-    float pad[6];                       // 26..31
+    float csx, snx, csy, sny;           // 0..3    scaled frame (PIEMD: cx1 a (cs, sn), cx1 b (cs, sn); SIS: cs, sn)
+    float k1, k2, k3, k4;               // 4..7    as HaloRec
+    float k5, ra, ra2c, rb;             // 8..11   2 sqrt(e) a, rcore, cx1^2 rcore^2, rcut
+    float rb2c;                         // 12      cx1^2 rcut^2
+    float hT, hc2, hs2m2, hs2h;         // 13..16  lab-frame amplitudes: trace, cos 2t, -2 sin 2t, sin 2t / 2
+    float near2;                        // 17
+    float sa, sb;                       // 18, 19  SIS: 1 - e, 1 + e
+    float pad[12];                      // 20..31
 };
 static_assert(sizeof(HaloRec32) == 128, "fp32 halo record is 128 bytes");
 

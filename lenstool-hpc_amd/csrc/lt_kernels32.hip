@@ -1,8 +1,8 @@
 // lt_kernels32.hip -- FP32-compensated Hessian (+ fused map epilogue) for gfx950.
 //
 // Same launch shape and run walk as the FP64 kernel (lt_kernels64.hip). Per (halo, pixel):
-//   * the halo-frame offset is formed from hi + lo float splits of the pixel and the centre, so it
-//     carries FP64 accuracy into FP32;
+//   * the lab-frame offset pixel - centre is formed in FP64 and rounded once to FP32 (the FP32
+//     offset is the exact offset to FP32 precision, however far the halo is from the origin);
 //   * inside the halo's near radius (HaloRec32::near2: cores, where the term is large) the FP64 term
 //     is used, outside it the FP32 term below;
 //   * FP32 terms are summed in FP32 over chunks of CHUNK halos and each chunk sum is added to FP64
@@ -44,23 +44,30 @@ struct Frame32 {
     float x, y;
 };
 
-// the row part (the same for every pixel of a thread): ty sin, ty cos
+// the row part (the same for every pixel of a thread): ty snx, ty csy, ty^2
 struct Row32 {
-    float tys, tyc;
+    float tys, tyc, ty2;
 };
 
-__device__ __forceinline__ Row32 row32(const HaloRec32& h, float pyh, float pyl)
+__device__ __forceinline__ Row32 row32(const HaloRec32& h, double yh, double py)
 {
-    const float ty = (pyh - h.yh_hi) + (pyl - h.yh_lo);
-    return Row32{ty * h.sn, ty * h.cs};
+    const float ty = (float)(py - yh);
+    return Row32{ty * h.snx, ty * h.csy, ty * ty};
 }
 
-__device__ __forceinline__ Frame32 frame32(const HaloRec32& h, const Row32& r, float pxh, float pxl)
+// lab-frame x offset of a pixel from the halo centre: the FP64 difference rounded once to FP32
+__device__ __forceinline__ float offset_x(double xh, double px)
 {
-    const float tx = (pxh - h.xh_hi) + (pxl - h.xh_lo);
+    return (float)(px - xh);
+}
+
+// halo-frame coordinates, scaled for PIEMD records (X = cx1 a x, Y = cx1 b y; lt_terms64.h) and
+// plain for SIS records (csx = csy = cos, snx = sny = sin)
+__device__ __forceinline__ Frame32 frame32(const HaloRec32& h, const Row32& r, float tx)
+{
     Frame32 f;
-    f.x = __builtin_fmaf(tx, h.cs, r.tys);
-    f.y = __builtin_fmaf(-tx, h.sn, r.tyc);
+    f.x = __builtin_fmaf(tx, h.csx, r.tys);
+    f.y = __builtin_fmaf(-tx, h.sny, r.tyc);
     return f;
 }
 
@@ -69,36 +76,35 @@ struct Q32 {
     float xx, xy, yy, iq;
 };
 
-// pixel-only quantities shared by the core and cut terms
+// pixel-only quantities shared by the core and cut terms (the scaled form of lt_terms64.h)
 struct QShared {
-    float wq0, Qi2, cxa2x2, b2y, e2a2x, cx1x, yicx1;
+    float wq0, Qi2, A, Y, Qi, kx2, ky4;
 };
 
-__device__ __forceinline__ QShared q_shared(const HaloRec32& h, float x, float y)
+__device__ __forceinline__ QShared q_shared(const HaloRec32& h, float X, float Y)
 {
     QShared s;
-    const float x2 = x * x;
-    s.wq0 = __builtin_fmaf(h.a2, x2, h.b2 * (y * y));
-    s.e2a2x = h.e2a2 * x;
-    s.Qi2 = s.e2a2x * s.e2a2x;
-    s.cxa2x2 = h.cxa2 * x2;
-    s.b2y = h.b2 * y;
-    s.cx1x = h.cx1 * x;
-    s.yicx1 = y * h.icx1;
+    s.Y = Y;
+    s.A = X * X;
+    s.wq0 = __builtin_fmaf(Y, Y, s.A);
+    s.Qi = h.k1 * X;
+    s.Qi2 = s.Qi * s.Qi;
+    s.kx2 = h.k2 * X;
+    s.ky4 = h.k4 * Y;
     return s;
 }
 
-__device__ __forceinline__ Q32 q_term(const HaloRec32& h, const QShared& s, float r, float r2)
+__device__ __forceinline__ Q32 q_term(const HaloRec32& h, const QShared& s, float r, float r2c)
 {
-    const float w2 = s.wq0 + r2;
-    const float iw = rsq32(w2);
-    const float w = w2 * iw;
-    const float Qr = __builtin_fmaf(h.cx1, w, r);
-    const float U = __builtin_fmaf(s.b2y, iw, h.e2ab);
+    const float W2 = s.wq0 + r2c;
+    const float iW = rsq32(W2);
+    const float W = W2 * iW;
+    const float Qr = W + r;
+    const float U = __builtin_fmaf(s.Y, iW, h.k5);
     Q32 o;
-    o.xx = __builtin_fmaf(-s.cxa2x2, iw, Qr);
-    o.xy = __builtin_fmaf(-s.cx1x, U, s.e2a2x);
-    o.yy = __builtin_fmaf(U, __builtin_fmaf(h.e2, w, -s.yicx1), Qr);
+    o.xx = __builtin_fmaf(-s.A, iW, Qr);
+    o.xy = __builtin_fmaf(-s.kx2, U, s.Qi);
+    o.yy = __builtin_fmaf(U, __builtin_fmaf(h.k3, W, -s.ky4), Qr);
     o.iq = rcp32(__builtin_fmaf(Qr, Qr, s.Qi2));
     return o;
 }
@@ -117,10 +123,10 @@ template <bool CUT>
 __device__ __forceinline__ void hess_piemd32(const HaloRec32& h, const Frame32& f, float& fT, float& fD, float& fB)
 {
     const QShared s = q_shared(h, f.x, f.y);
-    const Q32 a = q_term(h, s, h.ra, h.ra2);
+    const Q32 a = q_term(h, s, h.ra, h.ra2c);
     float xx, xy, yy;
     if (CUT) {
-        const Q32 b = q_term(h, s, h.rb, h.rb2);
+        const Q32 b = q_term(h, s, h.rb, h.rb2c);
         xx = __builtin_fmaf(a.xx, a.iq, -(b.xx * b.iq));
         xy = __builtin_fmaf(a.xy, a.iq, -(b.xy * b.iq));
         yy = __builtin_fmaf(a.yy, a.iq, -(b.yy * b.iq));
@@ -145,8 +151,7 @@ __device__ __forceinline__ void hess_sis32(const HaloRec32& h, const Frame32& f,
 // lanes) straight into the FP64 run sums.
 template <int TYPE, int PX>
 __device__ __forceinline__ void run32(const HaloRec* __restrict__ h64, const HaloRec32* __restrict__ h32, int start,
-                                      int end, const double* px, double py, const float* pxh, const float* pxl,
-                                      float pyh, float pyl, double* sT, double* sD, double* sB)
+                                      int end, const double* px, double py, double* sT, double* sD, double* sB)
 {
     for (int c0 = start; c0 < end; c0 += CHUNK) {
         const int c1 = min(c0 + CHUNK, end);
@@ -155,16 +160,19 @@ __device__ __forceinline__ void run32(const HaloRec* __restrict__ h64, const Hal
         for (int k = 0; k < PX; ++k) fT[k] = fD[k] = fB[k] = 0.f;
         for (int i = c0; i < c1; ++i) {
             const HaloRec32& h = h32[i];
-            const Row32 rw = row32(h, pyh, pyl);
+            const double xh = h64[i].xh;
+            const Row32 rw = row32(h, h64[i].yh, py);
 #pragma unroll
             for (int k = 0; k < PX; ++k) {
-                const Frame32 f = frame32(h, rw, pxh[k], pxl[k]);
-                if (__builtin_fmaf(f.x, f.x, f.y * f.y) < h.near2) {
+                const float tx = offset_x(xh, px[k]);
+                // the near-core test on the unrotated offset (the distance does not depend on the frame)
+                if (__builtin_fmaf(tx, tx, rw.ty2) < h.near2) {
                     const HaloRec& hd = h64[i];
                     if constexpr (TYPE == TYPE_DPIE) hess_piemd<true>(hd, px[k], py, sT[k], sD[k], sB[k]);
                     else if constexpr (TYPE == TYPE_PIEMD) hess_piemd<false>(hd, px[k], py, sT[k], sD[k], sB[k]);
                     else hess_sis(hd, px[k], py, sT[k], sD[k], sB[k]);
                 } else {
+                    const Frame32 f = frame32(h, rw, tx);
                     if constexpr (TYPE == TYPE_DPIE) hess_piemd32<true>(h, f, fT[k], fD[k], fB[k]);
                     else if constexpr (TYPE == TYPE_PIEMD) hess_piemd32<false>(h, f, fT[k], fD[k], fB[k]);
                     else hess_sis32(h, f, fT[k], fD[k], fB[k]);
@@ -192,15 +200,9 @@ __global__ __launch_bounds__(WAVE* ROWS) void k_hessian32(const HaloRec* __restr
     h32 += (int64_t)blockIdx.z * g.halo_stride;
     const int col0 = blockIdx.x * (WAVE * PX) + threadIdx.x;
     const double py = pixel_coord(g.ymin, row + g.jstart, g.dy);
-    const float pyh = (float)py, pyl = (float)(py - (double)pyh);
     double px[PX];
-    float pxh[PX], pxl[PX];
 #pragma unroll
-    for (int k = 0; k < PX; ++k) {
-        px[k] = pixel_coord(g.xmin, col0 + k * WAVE + g.istart, g.dx);
-        pxh[k] = (float)px[k];
-        pxl[k] = (float)(px[k] - (double)pxh[k]);
-    }
+    for (int k = 0; k < PX; ++k) px[k] = pixel_coord(g.xmin, col0 + k * WAVE + g.istart, g.dx);
     double tT[PX], tD[PX], tB[PX];
 #pragma unroll
     for (int k = 0; k < PX; ++k) tT[k] = tD[k] = tB[k] = 0.;
@@ -221,11 +223,11 @@ __global__ __launch_bounds__(WAVE* ROWS) void k_hessian32(const HaloRec* __restr
                 }
             }
         } else if (run.type == TYPE_DPIE) {
-            run32<TYPE_DPIE, PX>(h64, h32, run.start, end, px, py, pxh, pxl, pyh, pyl, sT, sD, sB);
+            run32<TYPE_DPIE, PX>(h64, h32, run.start, end, px, py, sT, sD, sB);
         } else if (run.type == TYPE_PIEMD) {
-            run32<TYPE_PIEMD, PX>(h64, h32, run.start, end, px, py, pxh, pxl, pyh, pyl, sT, sD, sB);
+            run32<TYPE_PIEMD, PX>(h64, h32, run.start, end, px, py, sT, sD, sB);
         } else {
-            run32<TYPE_SIS, PX>(h64, h32, run.start, end, px, py, pxh, pxl, pyh, pyl, sT, sD, sB);
+            run32<TYPE_SIS, PX>(h64, h32, run.start, end, px, py, sT, sD, sB);
         }
         const double w = run.weight;
 #pragma unroll

@@ -107,12 +107,21 @@ int pack_model(const lt_potential_soa* L, int n, Use use, std::vector<HaloRec>& 
             const double t05 = (t == TYPE_DPIE) ? rcut / (rcut - rc) : 1.0;
             h.ampg = b0 * t05 * ci;
             h.amph = b0 * t05 * ci;
-            h.hk1 = cx1 * h.e2 * h.kx;
-            h.hk2 = cx1 * h.e2;
-            h.hk3 = h.e2 * cx1 * h.ky;
-            h.hk4 = h.e2 * h.icx1;
-            h.hk5 = h.icx1 * h.icx1 + h.e2 * h.e2 * h.ky;
-            h.hk6 = h.e2 * h.ky * h.icx1;
+            // Hessian in scaled halo-frame coordinates X = cx1 a x, Y = cx1 b y (lt_terms64.h)
+            {
+                const double a = 1.0 / (1.0 + eps), b = 1.0 / (1.0 - eps);
+                h.csx = cx1 * a * cs;
+                h.snx = cx1 * a * sn;
+                h.csy = cx1 * b * cs;
+                h.sny = cx1 * b * sn;
+                h.k1 = h.e2 * a / cx1;
+                h.k2 = b / a;
+                h.k3 = h.e2 * b / cx1;
+                h.k4 = 1.0 / (cx1 * cx1);
+                h.k5 = h.e2 * a;
+                h.ra2c = cx1 * cx1 * rc * rc;
+                h.rb2c = cx1 * cx1 * rcut * rcut;
+            }
         } else {  // external shear: 3 eps (cos 2t, sin 2t; sin 2t, -cos 2t)
             h.h14a = 3.0 * eps * cos2;
             h.h14b = 3.0 * eps * sin2;
@@ -121,41 +130,34 @@ int pack_model(const lt_potential_soa* L, int n, Use use, std::vector<HaloRec>& 
         h.hc2 = h.amph * cos2;
         h.hs2 = h.amph * sin2;
         if (piemd) {
-            const double a = 1.0 / (1.0 + eps), b = 1.0 / (1.0 - eps);
-            h.e2ab = h.e2 * a * b;
-            h.e2a2 = h.e2 * a * a;
-            h.cxa2 = h.cx1 * a * a;
-            h.qT = h.amph * h.e2a2;
-            h.qc2 = h.hc2 * h.e2a2;
-            h.qs2m2 = -2.0 * h.hs2 * h.e2a2;
-            h.qs2h = 0.5 * h.hs2 * h.e2a2;
+            const double a = 1.0 / (1.0 + eps);
+            const double e2a2 = h.e2 * a * a;
+            h.qT = h.amph * e2a2;
+            h.qc2 = h.hc2 * e2a2;
+            h.qs2m2 = -2.0 * h.hs2 * e2a2;
+            h.qs2h = 0.5 * h.hs2 * e2a2;
         }
         recs[i] = h;
 
         HaloRec32& f = recs32[i];
         f = HaloRec32{};
-        f.xh_hi = (float)x;
-        f.xh_lo = (float)(x - (double)f.xh_hi);
-        f.yh_hi = (float)y;
-        f.yh_lo = (float)(y - (double)f.yh_hi);
-        f.cs = (float)cs;
-        f.sn = (float)sn;
         if (piemd) {
-            const double a = 1.0 / (1.0 + eps), b = 1.0 / (1.0 - eps), e2 = h.e2;
+            const double a = 1.0 / (1.0 + eps), e2 = h.e2;
             // The FP32 terms are the I_{0.5} Hessian times 1 / (2 sqrt(e) a^2): fold that factor back in.
             const double amp = h.amph * e2 * a * a;
-            f.a2 = (float)(a * a);
-            f.b2 = (float)(b * b);
-            f.e2 = (float)e2;
-            f.cx1 = (float)h.cx1;
-            f.e2ab = (float)(e2 * a * b);
-            f.e2a2 = (float)(e2 * a * a);
-            f.cxa2 = (float)(h.cx1 * a * a);
+            f.csx = (float)h.csx;
+            f.snx = (float)h.snx;
+            f.csy = (float)h.csy;
+            f.sny = (float)h.sny;
+            f.k1 = (float)h.k1;
+            f.k2 = (float)h.k2;
+            f.k3 = (float)h.k3;
+            f.k4 = (float)h.k4;
+            f.k5 = (float)h.k5;
             f.ra = (float)h.rc;
-            f.ra2 = (float)h.ra2;
+            f.ra2c = (float)h.ra2c;
             f.rb = (float)h.rcut;
-            f.rb2 = (float)h.rb2;
-            f.icx1 = (float)h.icx1;
+            f.rb2c = (float)h.rb2c;
             f.hT = (float)amp;
             f.hc2 = (float)(amp * cos2);
             f.hs2m2 = (float)(-2.0 * amp * sin2);
@@ -166,6 +168,8 @@ int pack_model(const lt_potential_soa* L, int n, Use use, std::vector<HaloRec>& 
             const double R = kFp32NearScalePiemd * std::fabs(b0 * t05);
             f.near2 = (float)std::fmin(R * R, 3.0e38);
         } else if (t == TYPE_SIS) {
+            f.csx = f.csy = (float)cs;
+            f.snx = f.sny = (float)sn;
             f.sa = (float)h.sa;
             f.sb = (float)h.sb;
             f.hT = (float)h.amph;

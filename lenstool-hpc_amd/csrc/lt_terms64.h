@@ -48,40 +48,49 @@ __device__ __forceinline__ PixelFrame halo_frame(const HaloRec& h, double px, do
 //   phi_yy ~ (Re Q + U t) / |Q|^2.
 // Re Q >= r > 0: unlike the closed form, whose two pieces are both singular at halo-frame
 // (0, 2 r s) and cancel there, nothing cancels anywhere. Returned: the numerators of xx, xy, yy
-// and 1 / |Q|^2.
+// and |Q|^2 (the caller forms the reciprocals: one for both radii of a dPIE).
 struct PiemdQ {
-    double xx, xy, yy, iq;
+    double xx, xy, yy, q2;
 };
 
+// Evaluated in scaled halo-frame coordinates X = cx1 a x, Y = cx1 b y (the scales are folded into
+// the rotation constants csx .. sny) with W = cx1 w: then W^2 = cx1^2 r^2 + X^2 + Y^2,
+// Re Q = W + r, and with U' = U / b = Y / W + 2 s a,
+//   xx = Re Q - X^2 / W,  xy = k1 X - k2 X U',  yy = Re Q + U' (k3 W - k4 Y),  Im Q = k1 X,
+// k1 = 2 s a / cx1, k2 = b / a, k3 = 2 s b / cx1, k4 = 1 / cx1^2, k5 = 2 s a: every pixel-dependent
+// quantity is one operation from X, Y and W.
 struct PiemdShared {
-    double wq0, Qi2, cxa2x2, b2y, e2a2x, cx1x, yicx1;
+    double wq0, Qi2, A, Y, Qi, kx2, ky4;
 };
 
-__device__ __forceinline__ PiemdShared piemd_shared(const HaloRec& h, const PixelFrame& f)
+__device__ __forceinline__ PiemdShared piemd_shared(const HaloRec& h, double px, double py)
 {
+    const double tx = px - h.xh, ty = py - h.yh;
+    const double X = __builtin_fma(tx, h.csx, ty * h.snx);
     PiemdShared s;
-    s.wq0 = __builtin_fma(h.kx, f.x2, h.ky * f.y2);
-    s.e2a2x = h.e2a2 * f.x;
-    s.Qi2 = s.e2a2x * s.e2a2x;
-    s.cxa2x2 = h.cxa2 * f.x2;
-    s.b2y = h.ky * f.y;
-    s.cx1x = h.cx1 * f.x;
-    s.yicx1 = f.y * h.icx1;
+    s.Y = __builtin_fma(ty, h.csy, -(tx * h.sny));
+    s.A = X * X;
+    s.wq0 = __builtin_fma(s.Y, s.Y, s.A);
+    s.Qi = h.k1 * X;
+    s.Qi2 = s.Qi * s.Qi;
+    s.kx2 = h.k2 * X;
+    s.ky4 = h.k4 * s.Y;
     return s;
 }
 
-__device__ __forceinline__ PiemdQ piemd_q_term(const HaloRec& h, const PiemdShared& s, double r, double r2)
+// one core radius r (r2c = cx1^2 r^2)
+__device__ __forceinline__ PiemdQ piemd_q_term(const HaloRec& h, const PiemdShared& s, double r, double r2c)
 {
-    const double w2 = s.wq0 + r2;
-    const double iw = rsq64(w2);
-    const double w = w2 * iw;
-    const double Qr = __builtin_fma(h.cx1, w, r);
-    const double U = __builtin_fma(s.b2y, iw, h.e2ab);
+    const double W2 = s.wq0 + r2c;
+    const double iW = rsq64_n2(W2);
+    const double W = W2 * iW;
+    const double Qr = W + r;
+    const double U = __builtin_fma(s.Y, iW, h.k5);
     PiemdQ o;
-    o.xx = __builtin_fma(-s.cxa2x2, iw, Qr);
-    o.xy = __builtin_fma(-s.cx1x, U, s.e2a2x);
-    o.yy = __builtin_fma(U, __builtin_fma(h.e2, w, -s.yicx1), Qr);
-    o.iq = rcp64(__builtin_fma(Qr, Qr, s.Qi2));
+    o.xx = __builtin_fma(-s.A, iW, Qr);
+    o.xy = __builtin_fma(-s.kx2, U, s.Qi);
+    o.yy = __builtin_fma(U, __builtin_fma(h.k3, W, -s.ky4), Qr);
+    o.q2 = __builtin_fma(Qr, Qr, s.Qi2);
     return o;
 }
 
@@ -89,20 +98,24 @@ template <bool CUT>
 __device__ __forceinline__ void hess_piemd(const HaloRec& h, double px, double py, double& aT, double& aD,
                                            double& aB)
 {
-    const PixelFrame f = halo_frame(h, px, py);
-    const PiemdShared s = piemd_shared(h, f);
-    const PiemdQ a = piemd_q_term(h, s, h.rc, h.ra2);
+    const PiemdShared s = piemd_shared(h, px, py);
+    const PiemdQ a = piemd_q_term(h, s, h.rc, h.ra2c);
     double xx, xy, yy;
     if (CUT) {
-        const PiemdQ b = piemd_q_term(h, s, h.rcut, h.rb2);
-        xx = __builtin_fma(a.xx, a.iq, -(b.xx * b.iq));
-        xy = __builtin_fma(a.xy, a.iq, -(b.xy * b.iq));
-        yy = __builtin_fma(a.yy, a.iq, -(b.yy * b.iq));
+        const PiemdQ b = piemd_q_term(h, s, h.rcut, h.rb2c);
+        // one reciprocal for both radii: 1/|Qa|^2 = |Qb|^2 / (|Qa|^2 |Qb|^2), and the same for b
+        const double ip = rcp64_n2(a.q2 * b.q2);
+        const double ia = b.q2 * ip, ib = a.q2 * ip;
+        xx = __builtin_fma(a.xx, ia, -(b.xx * ib));
+        xy = __builtin_fma(a.xy, ia, -(b.xy * ib));
+        yy = __builtin_fma(a.yy, ia, -(b.yy * ib));
     } else {
-        xx = a.xx * a.iq;
-        xy = a.xy * a.iq;
-        yy = a.yy * a.iq;
+        const double ia = rcp64_n2(a.q2);
+        xx = a.xx * ia;
+        xy = a.xy * ia;
+        yy = a.yy * ia;
     }
+    // xx, xy, yy are the numerators of the unscaled halo-frame derivatives (the scaling cancels)
     const double dT = xx + yy, dD = xx - yy, dB = xy;
     aT = __builtin_fma(h.qT, dT, aT);
     aD = __builtin_fma(h.qc2, dD, __builtin_fma(h.qs2m2, dB, aD));
@@ -116,7 +129,7 @@ __device__ __forceinline__ void hess_sis(const HaloRec& h, double px, double py,
 {
     PixelFrame f = halo_frame(h, px, py);
     double R2 = __builtin_fma(h.sa, f.x2, h.sb * f.y2);
-    double iR = rsq64(R2);
+    double iR = rsq64_n2(R2);
     double g = iR * iR * iR;
     double dyx = f.y2 - f.x2;
     double xy = f.x * f.y;
