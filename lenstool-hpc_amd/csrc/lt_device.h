@@ -37,7 +37,9 @@ struct alignas(64) HaloRec {
     double k1, k2, k3;              // 33..35 2 sqrt(e) a / cx1, b / a, 2 sqrt(e) b / cx1
     double qT, qc2, qs2m2, qs2h;    // 36..39 Hessian amplitudes of the division-free form
     double k4, k5;                  // 40, 41 1 / cx1^2, 2 sqrt(e) a   (a, b = 1/(1 +- e))
-    double pad[6];                  // 42..47
+    double pkp, pkq;                // 42, 43 potential: 2 sqrt(e) / (1 - e), 2 sqrt(e) / (1 + e)
+    double irc, ircut;              // 44, 45 1 / rcore, 1 / rcut
+    double pad[2];                  // 46, 47
 };
 static_assert(sizeof(HaloRec) == 384, "halo record is 384 bytes");
 

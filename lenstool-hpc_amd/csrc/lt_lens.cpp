@@ -122,6 +122,11 @@ int pack_model(const lt_potential_soa* L, int n, Use use, std::vector<HaloRec>& 
                 h.ra2c = cx1 * cx1 * rc * rc;
                 h.rb2c = cx1 * cx1 * rcut * rcut;
             }
+            // potential constants (lt_terms64.h pot_halo)
+            h.pkp = 2.0 * sqe / (1.0 - eps);
+            h.pkq = 2.0 * sqe / (1.0 + eps);
+            h.irc = 1.0 / rc;
+            h.ircut = (t == TYPE_DPIE) ? 1.0 / rcut : 0.0;
         } else {  // external shear: 3 eps (cos 2t, sin 2t; sin 2t, -cos 2t)
             h.h14a = 3.0 * eps * cos2;
             h.h14b = 3.0 * eps * sin2;

@@ -225,58 +225,64 @@ __device__ __forceinline__ void grad_sis(const HaloRec& h, double px, double py,
 // radius instead of five complex cosh / sinh, two complex logarithms, asinh and asin. The parts that
 // depend only on the pixel and the ellipticity are shared by the core and cut radii.
 struct PotShared {
-    double p, q, P, C, rem2;
+    double p, q, PC, pq, Nr, Ni, rem2, isrem, srem;
 };
 
-__device__ __forceinline__ PotShared pot_shared(double x, double y, double eps)
+// pixel quantities shared by both core radii (per-halo constants from the record: the divisions
+// and square roots of eps are the packer's)
+__device__ __forceinline__ PotShared pot_shared(const HaloRec& h, const PixelFrame& f)
 {
-    const double sqe = sqrt(eps);
-    const double iz = rsq64(__builtin_fma(x, x, y * y));
+    const double iz = rsq64(f.x2 + f.y2);
     PotShared s;
-    s.p = 2. * sqe / (1. - eps) * y * iz;
-    s.q = 2. * sqe / (1. + eps) * x * iz;
-    s.P = sqrt(__builtin_fma(s.p, s.p, 1.));
-    s.C = sqrt(__builtin_fma(-s.q, s.q, 1.));
-    s.rem2 = x * x / ((1. + eps) * (1. + eps)) + y * y / ((1. - eps) * (1. - eps));
+    s.p = h.pkp * f.y * iz;
+    s.q = h.pkq * f.x * iz;
+    const double P2 = __builtin_fma(s.p, s.p, 1.), C2 = __builtin_fma(-s.q, s.q, 1.);
+    const double P = P2 * rsq64(P2), C = C2 * rsq64(C2);
+    s.PC = P * C;
+    s.pq = s.p * s.q;
+    s.Nr = -s.p * C;
+    s.Ni = P * s.q;
+    s.rem2 = __builtin_fma(f.x2, h.kx, f.y2 * h.ky);
+    s.isrem = rsq64(s.rem2);
+    s.srem = s.rem2 * s.isrem;
     return s;
 }
 
 // ln(u / v) and arg(u / v) of two complex numbers: one logarithm, one atan2
 __device__ __forceinline__ void log_ratio(double ur, double ui, double vr, double vi, double& lr, double& li)
 {
-    lr = 0.5 * log(__builtin_fma(ur, ur, ui * ui) / __builtin_fma(vr, vr, vi * vi));
+    lr = 0.5 * log(__builtin_fma(ur, ur, ui * ui) * rcp64(__builtin_fma(vr, vr, vi * vi)));
     li = atan2(__builtin_fma(ui, vr, -(ur * vi)), __builtin_fma(ur, vr, ui * vi));
 }
 
-// pi05 of one core radius without the amplitude b0 ci: rc / rem ((c1 + c2).im x - (c1 + c2).re y)
-__device__ __forceinline__ double pi05_core(const PotShared& s, double x, double y, double rc)
+// pi05 of one core radius rc (irc = 1 / rc) without the amplitude b0 ci:
+// rc / rem ((c1 + c2).im x - (c1 + c2).re y)
+__device__ __forceinline__ double pi05_core(const PotShared& s, double x, double y, double rc, double irc)
 {
-    const double srem = sqrt(s.rem2);
-    const double W = sqrt(__builtin_fma(rc, rc, s.rem2));
-    const double PC = s.P * s.C, pq = s.p * s.q;
+    const double W2 = __builtin_fma(rc, rc, s.rem2);
+    const double W = W2 * rsq64(W2);
+    const double onePC = 1. + s.PC;
     double a1r, a1i, a2r, a2i;
-    log_ratio(1. + PC, -pq, PC + W / rc, -pq, a1r, a1i);
-    const double t = srem / (rc + W);
-    const double Nr = -s.p * s.C, Ni = s.P * s.q;
-    log_ratio(__builtin_fma(t, Nr, 1. + PC), __builtin_fma(t, Ni, -pq), __builtin_fma(-t, Nr, 1. + PC),
-              __builtin_fma(-t, Ni, -pq), a2r, a2i);
+    log_ratio(onePC, -s.pq, __builtin_fma(W, irc, s.PC), -s.pq, a1r, a1i);
+    const double t = s.srem * rcp64(rc + W);
+    log_ratio(__builtin_fma(t, s.Nr, onePC), __builtin_fma(t, s.Ni, -s.pq), __builtin_fma(-t, s.Nr, onePC),
+              __builtin_fma(-t, s.Ni, -s.pq), a2r, a2i);
     // c1 = sinh(2 eta) a1 with sinh 2 eta = N; c2 = (rem / rc) a2
-    const double sz = srem / rc;
-    const double cr = __builtin_fma(Nr, a1r, -(Ni * a1i)) + sz * a2r;
-    const double ci = __builtin_fma(Nr, a1i, Ni * a1r) + sz * a2i;
-    return rc / srem * __builtin_fma(ci, x, -(cr * y));
+    const double sz = s.srem * irc;
+    const double cr = __builtin_fma(s.Nr, a1r, -(s.Ni * a1i)) + sz * a2r;
+    const double ci = __builtin_fma(s.Nr, a1i, s.Ni * a1r) + sz * a2i;
+    return rc * s.isrem * __builtin_fma(ci, x, -(cr * y));
 }
 
 __device__ __forceinline__ double pot_halo(const HaloRec& h, int type, double px, double py)
 {
     PixelFrame f = halo_frame(h, px, py);
     if (type == TYPE_SIS) return h.b0 * sqrt(__builtin_fma(h.sa, f.x2, h.sb * f.y2));
-    const double amp = h.b0 * 0.5 * (1. - h.eps * h.eps) / sqrt(h.eps);
-    const PotShared s = pot_shared(f.x, f.y, h.eps);
-    const double pa = pi05_core(s, f.x, f.y, h.rc);
-    if (type == TYPE_PIEMD) return amp * pa;
-    const double ps = pi05_core(s, f.x, f.y, h.rcut);
-    return (h.rcut / (h.rcut - h.rc)) * amp * (pa - ps);
+    const PotShared s = pot_shared(h, f);
+    const double pa = pi05_core(s, f.x, f.y, h.rc, h.irc);
+    if (type == TYPE_PIEMD) return h.ampg * pa;
+    const double ps = pi05_core(s, f.x, f.y, h.rcut, h.ircut);
+    return h.ampg * (pa - ps);
 }
 
 // ------------------------------------------------------------------------------------------------
