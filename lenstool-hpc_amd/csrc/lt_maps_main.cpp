@@ -8,6 +8,9 @@
 //   lenstool_maps <file.par> [--bayes FILE] [--out DIR] [--precision fp64|fp32c] [--batch N]
 //                 [--samples N] [--quiet]
 //
+// --batch N: samples per launch (default 64, and no more than 4 GiB of planes per launch);
+// --samples N: the first N samples of the bayes file only.
+//
 // Output names follow the reference: <DIR>/<name>_<k>.fits for sample k, <DIR>/<name>.fits without
 // samples; displacement maps as <name1>_x / <name2>_y (with _<k> for samples: the reference writes
 // every sample's displacement map to the same file, only its last sample survives there).
@@ -195,7 +198,10 @@ int main(int argc, char** argv)
         const int prec = j.kind == LT_MAP_POT || j.kind == LT_MAP_DPL ? LT_PREC_FP64 : o.precision;
         const int nplanes = j.kind == LT_MAP_DPL ? 2 : 1;
         const long long npix = (long long)nc * nc;
-        const int bmax = std::min(o.batch, nsamples);
+        // samples per launch: --batch, and at most 4 GiB of map planes (device and host copy)
+        const long long plane_bytes = npix * (long long)sizeof(double) * nplanes;
+        const int bcap = (int)std::max(1LL, (4LL << 30) / plane_bytes);
+        const int bmax = std::min(std::min(o.batch, nsamples), bcap);
         std::vector<double*> dev(nplanes, nullptr);
         for (int p = 0; p < nplanes; ++p)
             if (fail(lt_dev_alloc((void**)&dev[p], npix * bmax * (long long)sizeof(double)), "lt_dev_alloc")) return 1;

@@ -332,6 +332,13 @@ class ChiBatch:
             pass
 
 
+def map_factors(kind, mode, cosmo, z_lens, z_source, dx=1.0, dy=1.0):
+    """(r, mult) arguments of Model.map for a reference map kind and mode (lt_map_factors)."""
+    r, mult = C.c_double(0), C.c_double(0)
+    check(lib().lt_map_factors(kind, mode, C.byref(cosmo), z_lens, z_source, dx, dy, C.byref(r), C.byref(mult)))
+    return r.value, mult.value
+
+
 def cosmo_dlsds(zl, zs, cosmo):
     return lib().lt_cosmo_dlsds(zl, zs, C.byref(cosmo))
 
