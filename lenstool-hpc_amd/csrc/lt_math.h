@@ -45,6 +45,101 @@ __device__ __forceinline__ double rsq64_n2(double q)
     return __builtin_fma(y * 0.5, e, y);
 }
 
+// Polynomial tables of atan_lt / log_lt (lenstool-hpc_amd/csrc/gen_poly.py derives them with
+// mpmath; the interpolation error is below 1e-17 relative in both functions).
+// kAtanP: degree 10 on [0, 0.171573], interpolation error 3.16e-17
+constexpr double kAtanP[11] = {
+    -0x1.5555555555555p-2,  // -0.3333333333333333016
+    0x1.999999999934cp-3,  // 0.19999999999995520678
+    -0x1.2492492436201p-3,  // -0.14285714284666542922
+    0x1.c71c71853d7fap-4,  // 0.11111111015256361727
+    -0x1.745d0b28a7e37p-4,  // -0.090909045781239018905
+    0x1.3b1263064f6b9p-4,  // 0.076921831908260865637
+    -0x1.10fa77b1a6d57p-4,  // -0.066645114473819480001
+    0x1.dfe6497e96323p-5,  // 0.058581489128022098816
+    -0x1.a0999c632b6edp-5,  // -0.050854497379402598613
+    0x1.4162c02b1dda3p-5,  // 0.039231658295587191303
+    -0x1.3a31b1c0fd3b7p-6,  // -0.019176887119062258989
+};
+// kLogR: degree 6 on [0, 0.0294373], interpolation error 1.55e-16
+constexpr double kLogR[7] = {
+    0x1.5555555555558p-2,  // 0.33333333333333348431
+    0x1.99999999952e2p-3,  // 0.19999999999949752245
+    0x1.2492492df148dp-3,  // 0.14285714312987742461
+    0x1.c71c62e5800a1p-4,  // 0.11111105567397539928
+    0x1.7462b4ab2ef6bp-4,  // 0.090914445626308610401
+    0x1.39fe606542ddep-4,  // 0.076658608002780206343
+    0x1.2b584aae78a57p-4,  // 0.073082248425217029302
+};
+
+// atan2(y, x) in FP64 for finite arguments, within a few units in the last place: the angle of
+// (max(|x|,|y|), min(|x|,|y|)) in [0, pi/4] is folded to [-pi/8, pi/8] by the rotation (u, v) ->
+// (u + v, v - u) (a rotation by -pi/4 times sqrt 2, which leaves the ratio v/u right), then
+// atan(t) = t + t^3 P(t^2) with |t| <= tan(pi/8) and one corrected reciprocal; the octant is put
+// back with pi/4, pi/2 - r, pi - r and the sign of y. atan2(0, 0) = 0 (pi for x = -0), NaN in ->
+// NaN out. About 40 instructions against the hundred of the library atan2.
+// one Horner step p z + c as a three-operand FMA: written out so that the coefficient c, which the
+// compiler keeps in a register across the halo loop, is read in place instead of being copied into
+// the accumulator of the two-operand form (one instruction per step instead of two)
+__device__ __forceinline__ double horner_step(double p, double z, double c)
+{
+    double r;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(p), "v"(z), "v"(c));
+    return r;
+}
+
+__device__ __forceinline__ double atan_lt_poly(double t)
+{
+    const double z = t * t;
+    double p = kAtanP[10];
+#pragma unroll
+    for (int k = 9; k >= 0; --k) p = horner_step(p, z, kAtanP[k]);
+    return __builtin_fma(t * z, p, t);
+}
+
+__device__ __forceinline__ double atan2_lt(double y, double x)
+{
+    constexpr double kPi = 3.141592653589793116, kPi2 = 1.5707963267948965580, kPi4 = 0.78539816339744827900;
+    constexpr double kTanPi8 = 0.41421356237309503;
+    const double ax = __builtin_fabs(x), ay = __builtin_fabs(y);
+    const double mx = ax > ay ? ax : ay, mn = ax > ay ? ay : ax;
+    const bool rot = mn > kTanPi8 * mx;
+    const double u = rot ? mx + mn : mx;
+    const double v = rot ? mn - mx : mn;
+    double r = atan_lt_poly(v * rcp64(u));
+    r = rot ? r + kPi4 : r;
+    r = ay > ax ? kPi2 - r : r;
+    r = __builtin_signbit(x) ? kPi - r : r;
+    r = mx == 0. ? (__builtin_signbit(x) ? kPi : 0.) : r;
+    r = __builtin_copysign(r, y);
+    return __builtin_isunordered(x, y) ? x + y : r;
+}
+
+// natural logarithm in FP64 for finite x > 0, within a few units in the last place:
+// x = m 2^e with m in [sqrt(1/2), sqrt 2), log m = 2 atanh(s) = 2 s + 2 s^3 R(s^2) with
+// s = (m - 1)/(m + 1) (|s| <= 0.1716, one corrected reciprocal), plus e ln 2 with ln 2 split so that
+// e ln2_hi is exact. About 30 instructions against the hundred of the library log.
+__device__ __forceinline__ double log_lt(double x)
+{
+    constexpr double kLn2Hi = 0x1.62e42fee00000p-1, kLn2Lo = 0x1.a39ef35793c76p-33;
+    constexpr double kSqrtHalf = 0.70710678118654757274;
+    int e;
+    double m = __builtin_frexp(x, &e);
+    const bool lo = m < kSqrtHalf;
+    m = lo ? m + m : m;
+    e = lo ? e - 1 : e;
+    const double f = m - 1.;
+    const double sv = f * rcp64(m + 1.);
+    const double w = sv * sv;
+    double p = kLogR[6];
+#pragma unroll
+    for (int k = 5; k >= 0; --k) p = horner_step(p, w, kLogR[k]);
+    const double s2 = sv + sv;
+    const double lm = __builtin_fma(s2 * w, p, s2);
+    const double de = (double)e;
+    return __builtin_fma(de, kLn2Hi, __builtin_fma(de, kLn2Lo, lm));
+}
+
 // Pixel coordinate exactly as the reference CPU forms it: xmin + (i * d), two roundings, no fused
 // multiply-add (src/grid_gradient_CPU.cpp:89-90).
 __device__ __forceinline__ double pixel_coord(double xmin, int i, double d)

@@ -188,8 +188,8 @@ __device__ __forceinline__ void grad_piemd(const HaloRec& h, double px, double p
         num = a.den1;
         den = a.den2;
     }
-    double arg = atan2(pim, pre);
-    double lnr = 0.5 * log(num * rcp64(den));
+    double arg = atan2_lt(pim, pre);
+    double lnr = 0.5 * log_lt(num * rcp64(den));
     double re = h.ampg * arg;
     double im = -h.ampg * lnr;
     gx += __builtin_fma(re, h.cs, -(im * h.sn));
@@ -251,8 +251,8 @@ __device__ __forceinline__ PotShared pot_shared(const HaloRec& h, const PixelFra
 // ln(u / v) and arg(u / v) of two complex numbers: one logarithm, one atan2
 __device__ __forceinline__ void log_ratio(double ur, double ui, double vr, double vi, double& lr, double& li)
 {
-    lr = 0.5 * log(__builtin_fma(ur, ur, ui * ui) * rcp64(__builtin_fma(vr, vr, vi * vi)));
-    li = atan2(__builtin_fma(ui, vr, -(ur * vi)), __builtin_fma(ur, vr, ui * vi));
+    lr = 0.5 * log_lt(__builtin_fma(ur, ur, ui * ui) * rcp64(__builtin_fma(vr, vr, vi * vi)));
+    li = atan2_lt(__builtin_fma(ui, vr, -(ur * vi)), __builtin_fma(ur, vr, ui * vi));
 }
 
 // pi05 of one core radius rc (irc = 1 / rc) without the amplitude b0 ci:
