@@ -188,6 +188,40 @@ __device__ __forceinline__ void run32(const HaloRec* __restrict__ h64, const Hal
     }
 }
 
+// The FP64 Hessian sums of one pixel over every run, exactly as the FP64 kernel forms them (same
+// terms, same order): the fallback for pixels where det A would amplify the FP32 rounding.
+__device__ __forceinline__ void hessian_fp64_pixel(const HaloRec* __restrict__ h64, const Run* __restrict__ runs,
+                                                int nruns, double px, double py, double& tT, double& tD, double& tB)
+{
+    tT = tD = tB = 0.;
+    for (int r = 0; r < nruns; ++r) {
+        const Run run = runs[r];
+        double sT = 0., sD = 0., sB = 0.;
+        const int end = run.start + run.count;
+        for (int i = run.start; i < end; ++i) {
+            const HaloRec& h = h64[i];
+            if (run.type == TYPE_DPIE) hess_piemd<true>(h, px, py, sT, sD, sB);
+            else if (run.type == TYPE_PIEMD) hess_piemd<false>(h, px, py, sT, sD, sB);
+            else if (run.type == TYPE_SIS) hess_sis(h, px, py, sT, sD, sB);
+            else {
+                sT += h.h14a + h.h14c;
+                sD += h.h14a - h.h14c;
+                sB += h.h14b;
+            }
+        }
+        tT = __builtin_fma(run.weight, sT, tT);
+        tD = __builtin_fma(run.weight, sD, tD);
+        tB = __builtin_fma(run.weight, sB, tB);
+    }
+}
+
+// det A = (1 - kappa)^2 - gamma^2 turns an error e in kappa and gamma into up to
+// 2 (|1 - kappa| + gamma) e: the FP32 terms leave e below about 4e-8, so where |1 - kappa| + gamma
+// exceeds this bound (next to singular or small cores: kappa reaches hundreds at a tenth of an
+// arcsecond from an isothermal centre) the pixel's sums are recomputed in FP64, keeping
+// |det A (FP32C) - det A (FP64)| under 1e-6 everywhere. A few pixels per halo core pay it.
+constexpr double kDetGuard = 8.0;
+
 template <int PX, int KIND>
 __global__ __launch_bounds__(WAVE* ROWS) void k_hessian32(const HaloRec* __restrict__ h64,
                                                           const HaloRec32* __restrict__ h32,
@@ -235,6 +269,14 @@ __global__ __launch_bounds__(WAVE* ROWS) void k_hessian32(const HaloRec* __restr
             tT[k] = __builtin_fma(w, sT[k], tT[k]);
             tD[k] = __builtin_fma(w, sD[k], tD[k]);
             tB[k] = __builtin_fma(w, sB[k], tB[k]);
+        }
+    }
+    if constexpr (KIND == LT_MAP_FIELDS || KIND == LT_MAP_AMPLI || KIND == LT_MAP_SHEAR) {
+        // one pixel at a time (not unrolled) so that the FP64 loop adds no registers to the kernel
+        for (int k = 0; k < PX; ++k) {
+            const double kap = 0.5 * tT[k] * o.r;
+            const double gb = 0.5 * (__builtin_fabs(tD[k]) + 2. * __builtin_fabs(tB[k])) * __builtin_fabs(o.r);
+            if (__builtin_fabs(1. - kap) + gb > kDetGuard) hessian_fp64_pixel(h64, runs, nruns, px[k], py, tT[k], tD[k], tB[k]);
         }
     }
 #pragma unroll

@@ -29,11 +29,11 @@ int main()
     }
     double *dy, *dx, *da, *dl;
     if (hipMalloc(&dy, n * 8) || hipMalloc(&dx, n * 8) || hipMalloc(&da, n * 8) || hipMalloc(&dl, n * 8)) return 1;
-    hipMemcpy(dy, y.data(), n * 8, hipMemcpyHostToDevice);
-    hipMemcpy(dx, x.data(), n * 8, hipMemcpyHostToDevice);
+    if (hipMemcpy(dy, y.data(), n * 8, hipMemcpyHostToDevice) || hipMemcpy(dx, x.data(), n * 8, hipMemcpyHostToDevice))
+        return 1;
     hipLaunchKernelGGL(k, dim3((n + 255) / 256), dim3(256), 0, 0, dy, dx, da, dl, n);
-    hipMemcpy(at.data(), da, n * 8, hipMemcpyDeviceToHost);
-    hipMemcpy(lg.data(), dl, n * 8, hipMemcpyDeviceToHost);
+    if (hipMemcpy(at.data(), da, n * 8, hipMemcpyDeviceToHost) || hipMemcpy(lg.data(), dl, n * 8, hipMemcpyDeviceToHost))
+        return 1;
     double wa = 0, wl = 0, wl_abs = 0;
     for (int i = 0; i < n; ++i) {
         long double ra = atan2l((long double)y[i], (long double)x[i]);
