@@ -272,8 +272,9 @@ __global__ __launch_bounds__(WAVE* ROWS) void k_hessian32(const HaloRec* __restr
         }
     }
     if constexpr (KIND == LT_MAP_FIELDS || KIND == LT_MAP_AMPLI || KIND == LT_MAP_SHEAR) {
-        // one pixel at a time (not unrolled) so that the FP64 loop adds no registers to the kernel
-        for (int k = 0; k < PX; ++k) {
+        // one pixel at a time (not unrolled) so that the FP64 loop adds no registers to the kernel; with
+        // no runs there is nothing to recompute
+        for (int k = 0; k < PX && nruns > 0; ++k) {
             const double kap = 0.5 * tT[k] * o.r;
             const double gb = 0.5 * (__builtin_fabs(tD[k]) + 2. * __builtin_fabs(tB[k])) * __builtin_fabs(o.r);
             if (__builtin_fabs(1. - kap) + gb > kDetGuard) hessian_fp64_pixel(h64, runs, nruns, px[k], py, tT[k], tD[k], tB[k]);
