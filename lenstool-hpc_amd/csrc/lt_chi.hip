@@ -72,6 +72,7 @@ int batch_alloc(lt_chi_batch_s* b, const int* nimages, const lt_galaxy* images)
     size_t o_ix = c.take<double>(I), o_iy = c.take<double>(I), o_idr = c.take<double>(I);
     size_t o_igx = c.take<double>(S * I), o_igy = c.take<double>(S * I);
     size_t o_sx = c.take<double>(S * K), o_sy = c.take<double>(S * K), o_sdr = c.take<double>(S * K);
+    size_t o_sb = c.take<double>(S * 3);
     size_t o_cnt = c.take<int32_t>(S * K), o_hits = c.take<int32_t>(S * K * DELENSE_CAP);
     size_t o_px = c.take<double>(S * K * LT_MAXIMPERSOURCE), o_py = c.take<double>(S * K * LT_MAXIMPERSOURCE);
     size_t o_loc = c.take<int32_t>(S * K), o_st = c.take<int32_t>(S), o_terms = c.take<double>(S * I);
@@ -107,6 +108,7 @@ int batch_alloc(lt_chi_batch_s* b, const int* nimages, const lt_galaxy* images)
     d.src_x = (double*)(p + o_sx);
     d.src_y = (double*)(p + o_sy);
     d.src_dr = (double*)(p + o_sdr);
+    d.src_bound = (double*)(p + o_sb);
     d.count = (int32_t*)(p + o_cnt);
     d.hits = (int32_t*)(p + o_hits);
     d.pos_x = (double*)(p + o_px);

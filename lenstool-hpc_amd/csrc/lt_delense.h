@@ -27,6 +27,7 @@ struct DelenseDesc {
     double* src_x;               // [nsamples*nsets] source barycentres
     double* src_y;
     double* src_dr;
+    double* src_bound;           // [nsamples*3] max |x|, |y|, min dr, max dr of the sample's sources
     int32_t* count;              // [nsamples*nsets] hits found
     int32_t* hits;               // [nsamples*nsets*DELENSE_CAP] cell*2 + (lower triangle)
     double* pos_x;               // [nsamples*nsets*LT_MAXIMPERSOURCE] image barycentres

@@ -13,41 +13,11 @@
 
 #include "lenstool_hip/lt_status.h"
 #include "lt_delense.h"
+#include "lt_delense_geom.h"
 #include "lt_device.h"
 #include "lt_math.h"
 
 namespace lt {
-
-struct P2 {
-    double x, y;
-};
-
-// scalar triple product of (A,1), (B,1), (C,1) (src/delense_CPU_utils.hpp:110-117)
-__device__ __forceinline__ double tri_det(P2 A, P2 B, P2 C)
-{
-#pragma clang fp contract(off)
-    return (B.x * C.y - B.y * C.x + A.x * B.y - A.y * B.x + A.y * C.x - A.x * C.y);
-}
-
-// inside-or-on-border test (src/delense_CPU_utils.hpp:125-142; :180-193 is the same test)
-__device__ __forceinline__ bool tri_inside(P2 P, P2 a, P2 b, P2 c)
-{
-#pragma clang fp contract(off)
-    double d = tri_det(a, b, c);
-    if (tri_det(a, b, P) * d < 0.) return false;
-    if (tri_det(b, c, P) * d < 0.) return false;
-    if (tri_det(c, a, P) * d < 0.) return false;
-    return true;
-}
-
-__device__ __forceinline__ P2 to_source(double x, double y, double dr, double gx, double gy)
-{
-#pragma clang fp contract(off)
-    P2 s;
-    s.x = x - dr * gx;
-    s.y = y - dr * gy;
-    return s;
-}
 
 // ------------------------------------------------------------------------------------------------
 // source barycentres: one thread per (sample, family); images in order, sum then divide
@@ -72,52 +42,125 @@ __global__ void k_barycentres(const double* __restrict__ img_gx, const double* _
     d.src_dr[o] = d.img_dr[first];
 }
 
-// Which triangle of cell (xid, yid) holds source P after delensing: 0 none, 1 upper, 2 lower
-// (src/delense_CPU.cpp:60-112: the upper triangle is tested first; the lower only if it misses).
-__device__ __forceinline__ int cell_hit(const double* __restrict__ gx, const double* __restrict__ gy,
-                                        const DelenseDesc& d, int xid, int yid, double px, double py, double dr)
+// corners of cell (xid, yid) of a sample's gradient planes, image-plane coordinates as the reference
+// forms them (src/delense_CPU.cpp:64-85)
+__device__ __forceinline__ CellCorners load_corners(const double* __restrict__ gx, const double* __restrict__ gy,
+                                                    const DelenseDesc& d, int xid, int yid)
 {
 #pragma clang fp contract(off)
     const int n = d.n;
     const int64_t g = (int64_t)yid * n + xid;
-    const double xp = d.xmin + xid * d.dx;
-    const double yp = d.ymin + yid * d.dy;
-    const double xq = xp + d.dx, yq = yp + d.dy;
-    const P2 P = {px, py};
-    // gradient at the four corners: g (x,y), g+n (x,y+dy), g+1 (x+dx,y), g+n+1 (x+dx,y+dy)
-    // upper triangle a=(x,y) b=(x,y+dy) c=(x+dx,y)
-    const P2 ua = to_source(xp, yp, dr, gx[g], gy[g]);
-    const P2 ub = to_source(xp, yq, dr, gx[g + n], gy[g + n]);
-    const P2 uc = to_source(xq, yp, dr, gx[g + 1], gy[g + 1]);
-    if (tri_inside(P, ua, ub, uc)) return 1;
-    // lower triangle a=(x+dx,y+dy) b=(x+dx,y) c=(x,y+dy)
-    const P2 la = to_source(xq, yq, dr, gx[g + n + 1], gy[g + n + 1]);
-    const P2 lb = to_source(xq, yp, dr, gx[g + 1], gy[g + 1]);
-    const P2 lc = to_source(xp, yq, dr, gx[g + n], gy[g + n]);
-    return tri_inside(P, la, lb, lc) ? 2 : 0;
+    CellCorners c;
+    c.xp = d.xmin + xid * d.dx;
+    c.yp = d.ymin + yid * d.dy;
+    c.xq = c.xp + d.dx;
+    c.yq = c.yp + d.dy;
+    c.g0x = gx[g];
+    c.g0y = gy[g];
+    c.g1x = gx[g + n];
+    c.g1y = gy[g + n];
+    c.g2x = gx[g + 1];
+    c.g2y = gy[g + 1];
+    c.g3x = gx[g + n + 1];
+    c.g3y = gy[g + n + 1];
+    return c;
 }
 
 // ------------------------------------------------------------------------------------------------
-// delensing scan: one thread per (cell, sample), all sources; hits appended per (sample, source)
+// per-sample bounds of the sources for the scan's cull: max |x|, |y| and the range of D_ls/D_s; a
+// non-finite source makes the bounds NaN, which switches the cull off for the whole sample
 // ------------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_delense(const double* __restrict__ gx, const double* __restrict__ gy,
-                                                 DelenseDesc d)
+__global__ void k_source_bounds(DelenseDesc d)
 {
-#pragma clang fp contract(off)
-    const int n = d.n;
-    const int xid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int yid = blockIdx.y;
-    const int s = blockIdx.z;
-    if (xid >= n - 1 || yid >= n - 1) return;
-    const int64_t base = (int64_t)s * d.grid_stride;
-    const int cell = yid * (n - 1) + xid;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= d.nsamples) return;
+    double pmax = 0., lo = 0., hi = 0., sum = 0.;  // NaN and infinity reach the sum (fmin/fmax drop NaN)
     for (int k = 0; k < d.nsets; ++k) {
         const int64_t sk = (int64_t)s * d.nsets + k;
-        const int hit = cell_hit(gx + base, gy + base, d, xid, yid, d.src_x[sk], d.src_y[sk], d.src_dr[sk]);
-        if (hit) {
-            const int slot = atomicAdd(&d.count[sk], 1);
-            if (slot < DELENSE_CAP) d.hits[sk * DELENSE_CAP + slot] = cell * 2 + (hit - 1);
+        const double x = fabs(d.src_x[sk]), y = fabs(d.src_y[sk]), r = d.src_dr[sk];
+        sum += x + y + fabs(r);
+        pmax = fmax(pmax, fmax(x, y));
+        lo = k ? fmin(lo, r) : r;
+        hi = k ? fmax(hi, r) : r;
+    }
+    const bool finite = sum <= 1e300 && pmax <= 1e100 && fabs(lo) <= 1e100 && fabs(hi) <= 1e100;
+    const double nan = __builtin_nan("");
+    d.src_bound[3 * (int64_t)s] = finite ? pmax : nan;
+    d.src_bound[3 * (int64_t)s + 1] = finite ? lo : nan;
+    d.src_bound[3 * (int64_t)s + 2] = finite ? hi : nan;
+}
+
+// append a hit of source k of sample s in cell `cell` (0 = none, 1 upper, 2 lower triangle)
+__device__ __forceinline__ void record_hit(const DelenseDesc& d, int s, int k, int cell, int hit)
+{
+    if (!hit) return;
+    const int64_t sk = (int64_t)s * d.nsets + k;
+    const int slot = atomicAdd(&d.count[sk], 1);
+    if (slot < DELENSE_CAP) d.hits[sk * DELENSE_CAP + slot] = cell * 2 + (hit - 1);
+}
+
+// ------------------------------------------------------------------------------------------------
+// delensing scan: one thread per (cell, sample), all sources (staged in LDS, SCAN_BLOCK per pass);
+// hits appended per (sample, source)
+// ------------------------------------------------------------------------------------------------
+constexpr int SCAN_BLOCK = 256;  // cells per block along a row; also the sources staged in LDS per pass
+constexpr int SCAN_UNROLL = 4;   // sources culled per step of the scan loop
+
+// One pass over cnt staged sources. SCAN_UNROLL sources per step: one margin (the smallest of their
+// culls) decides the step in straight-line code; the reference test runs only where a cull fails (a
+// few cells around each image, degenerate or non-finite cells), one source at a time. RHO_ONLY: every
+// lane's cell has d_ok (or is past the row end), so the margin needs only the distance.
+template <bool RHO_ONLY>
+__device__ __forceinline__ void scan_pass(const DelenseDesc& d, const CellCorners& c, const CellFilter& f, bool active,
+                                          int s, int k0, int cell, int cnt, const double* lsx, const double* lsy,
+                                          const double* ldr)
+{
+    int j0 = 0;
+    for (; j0 + SCAN_UNROLL <= cnt; j0 += SCAN_UNROLL) {
+        double mg = RHO_ONLY ? far_margin_rho(f, c, lsx[j0], lsy[j0], ldr[j0]) : far_margin(f, c, lsx[j0], lsy[j0], ldr[j0]);
+#pragma unroll
+        for (int u = 1; u < SCAN_UNROLL; ++u)
+            mg = fmin(mg, RHO_ONLY ? far_margin_rho(f, c, lsx[j0 + u], lsy[j0 + u], ldr[j0 + u])
+                                   : far_margin(f, c, lsx[j0 + u], lsy[j0 + u], ldr[j0 + u]));
+        if (!active | (f.on & (mg > 0.))) continue;
+#pragma unroll 1
+        for (int j = j0; j < j0 + SCAN_UNROLL; ++j)
+            if (!far_from_cell(f, c, lsx[j], lsy[j], ldr[j])) record_hit(d, s, k0 + j, cell, cell_hit(c, lsx[j], lsy[j], ldr[j]));
+    }
+#pragma unroll 1
+    for (; j0 < cnt; ++j0)
+        if (active && !far_from_cell(f, c, lsx[j0], lsy[j0], ldr[j0]))
+            record_hit(d, s, k0 + j0, cell, cell_hit(c, lsx[j0], lsy[j0], ldr[j0]));
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_delense(const double* __restrict__ gx, const double* __restrict__ gy,
+                                                        DelenseDesc d)
+{
+    __shared__ double lsx[SCAN_BLOCK], lsy[SCAN_BLOCK], ldr[SCAN_BLOCK];
+    const int n = d.n;
+    const int xid = blockIdx.x * SCAN_BLOCK + threadIdx.x;
+    const int yid = blockIdx.y;  // the grid has n - 1 rows of blocks
+    const int s = blockIdx.z;
+    const bool active = xid < n - 1;
+    const int64_t base = (int64_t)s * d.grid_stride;
+    const int cell = yid * (n - 1) + xid;
+    // lanes past the row end read the last cell (in bounds) and record nothing
+    const CellCorners c = load_corners(gx + base, gy + base, d, active ? xid : n - 2, yid);
+    const double* sb = d.src_bound + 3 * (int64_t)s;
+    const CellFilter f = cell_filter(c, sb[0], sb[1], sb[2]);
+    const bool rho_only = __all(!active || f.d_ok);  // uniform over the wavefront
+    for (int k0 = 0; k0 < d.nsets; k0 += SCAN_BLOCK) {
+        const int cnt = min(SCAN_BLOCK, d.nsets - k0);
+        __syncthreads();  // the previous pass is done with the staged sources
+        if ((int)threadIdx.x < cnt) {
+            const int64_t sk = (int64_t)s * d.nsets + k0 + threadIdx.x;
+            lsx[threadIdx.x] = d.src_x[sk];
+            lsy[threadIdx.x] = d.src_y[sk];
+            ldr[threadIdx.x] = d.src_dr[sk];
         }
+        __syncthreads();
+        if (rho_only) scan_pass<true>(d, c, f, active, s, k0, cell, cnt, lsx, lsy, ldr);
+        else scan_pass<false>(d, c, f, active, s, k0, cell, cnt, lsx, lsy, ldr);
     }
 }
 
@@ -142,7 +185,8 @@ __global__ void k_delense_finish(const double* __restrict__ gx, const double* __
         int got = 0;
         for (int yid = 0; yid < d.n - 1 && got < LT_MAXIMPERSOURCE; ++yid)
             for (int xid = 0; xid < d.n - 1 && got < LT_MAXIMPERSOURCE; ++xid) {
-                const int hit = cell_hit(gx + base, gy + base, d, xid, yid, d.src_x[sk], d.src_y[sk], d.src_dr[sk]);
+                const int hit = cell_hit(load_corners(gx + base, gy + base, d, xid, yid), d.src_x[sk], d.src_y[sk],
+                                         d.src_dr[sk]);
                 if (hit) h[got++] = (yid * (d.n - 1) + xid) * 2 + (hit - 1);
             }
         cnt = got;
@@ -279,10 +323,13 @@ hipError_t launch_barycentres(const double* img_gx, const double* img_gy, const 
 
 hipError_t launch_delense(const double* gx, const double* gy, const DelenseDesc& d, hipStream_t s)
 {
-    dim3 blk(256);
-    dim3 grd((unsigned)((d.n - 1 + 255) / 256), (unsigned)(d.n - 1), (unsigned)d.nsamples);
-    hipLaunchKernelGGL(k_delense, grd, blk, 0, s, gx, gy, d);
+    hipLaunchKernelGGL(k_source_bounds, dim3((d.nsamples + 63) / 64), dim3(64), 0, s, d);
     hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    dim3 blk(SCAN_BLOCK);
+    dim3 grd((unsigned)((d.n - 1 + SCAN_BLOCK - 1) / SCAN_BLOCK), (unsigned)(d.n - 1), (unsigned)d.nsamples);
+    hipLaunchKernelGGL(k_delense, grd, blk, 0, s, gx, gy, d);
+    e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_delense_finish, dim3((d.nsets + 63) / 64, d.nsamples), dim3(64), 0, s, gx, gy, d);
     return hipGetLastError();

@@ -1,0 +1,239 @@
+// Host check of the delensing scan's cull (lenstool-hpc_amd/csrc/lt_delense_geom.h), the same header
+// the device kernel compiles:
+//   1. soundness: whenever far_from_cell() (near or far setting) or the fast path's margins
+//      (on && far_margin() > 0, d_ok && far_margin_rho() > 0) say "skip", the reference test
+//      cell_hit() finds no hit, and d_ok implies |exact det| > 3.2e-11 M^2 for both triangles;
+//   2. the two error bounds its proof rests on, against __float128 arithmetic:
+//      |tri_det computed - exact| <= 4.0e-15 M^2 and |d(dr) polynomial - exact det of the computed
+//      corners| <= 3.2e-14 M^2 (M = largest coordinate involved);
+//   3. efficiency: on lens-like cells (0.05" or larger) far from the source the fast path's margin
+//      decides almost every pair.
+// Cells come from smooth lens-like gradient fields, near-collinear and collapsed mapped triangles,
+// wild and huge gradients, sources on corners and edges, and NaN / infinity in gradients, sources and
+// D_ls/D_s. Exit status 0 = all checks hold; the counts are printed.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <limits>
+#include <random>
+
+#include "lt_delense_geom.h"
+
+using lt::CellCorners;
+using lt::CellFilter;
+using lt::P2;
+
+typedef __float128 q;
+
+static q qdet(P2 a, P2 b, P2 c)
+{
+    return ((q)b.x - a.x) * ((q)c.y - a.y) - ((q)b.y - a.y) * ((q)c.x - a.x);
+}
+
+static double amax(double a, double b) { return std::fabs(a) > std::fabs(b) ? std::fabs(a) : std::fabs(b); }
+
+struct Stats {
+    long trials = 0, culled = 0, hits = 0, unsound = 0, det_bad = 0, poly_bad = 0, far_pairs = 0, far_culled = 0,
+         dok = 0, dok_bad = 0;
+    double det_worst = 0, poly_worst = 0;
+};
+
+static void check(Stats& st, const CellCorners& c, double px, double py, double dr, double pmax, double drlo,
+                  double drhi, bool lens_like_far)
+{
+    const CellFilter f = lt::cell_filter(c, pmax, drlo, drhi);
+    const bool far = lt::far_from_cell(f, c, px, py, dr);
+    const bool fast = (f.on && lt::far_margin(f, c, px, py, dr) > 0.) ||
+                      (f.d_ok && lt::far_margin_rho(f, c, px, py, dr) > 0.);
+    const int hit = lt::cell_hit(c, px, py, dr);
+    ++st.trials;
+    st.culled += far;
+    st.dok += f.d_ok;
+    st.hits += hit != 0;
+    if ((far || fast) && hit) {
+        if (st.unsound < 5)
+            std::printf("UNSOUND: corners (%.17g %.17g %.17g %.17g) g (%.17g %.17g | %.17g %.17g | %.17g %.17g | %.17g "
+                        "%.17g) P (%.17g %.17g) dr %.17g\n",
+                        c.xp, c.yp, c.xq, c.yq, c.g0x, c.g0y, c.g1x, c.g1y, c.g2x, c.g2y, c.g3x, c.g3y, px, py, dr);
+        ++st.unsound;
+    }
+    if (lens_like_far) {
+        ++st.far_pairs;
+        st.far_culled += fast;
+    }
+    if (!f.on || !std::isfinite(px) || !std::isfinite(py) || !std::isfinite(dr)) return;
+    // the error bounds behind the proof, on the corners the reference computes
+    const P2 v0 = lt::to_source(c.xp, c.yp, dr, c.g0x, c.g0y);
+    const P2 v1 = lt::to_source(c.xp, c.yq, dr, c.g1x, c.g1y);
+    const P2 v2 = lt::to_source(c.xq, c.yp, dr, c.g2x, c.g2y);
+    const P2 v3 = lt::to_source(c.xq, c.yq, dr, c.g3x, c.g3y);
+    const P2 P = {px, py};
+    double m = amax(amax(amax(v0.x, v0.y), amax(v1.x, v1.y)), amax(amax(v2.x, v2.y), amax(v3.x, v3.y)));
+    m = amax(m, amax(px, py));
+    const q m2 = (q)m * m;
+    const P2 tris[2][3] = {{v0, v1, v2}, {v3, v2, v1}};
+    for (const auto& t : tris) {
+        const P2 probes[4][3] = {{t[0], t[1], t[2]}, {t[0], t[1], P}, {t[1], t[2], P}, {t[2], t[0], P}};
+        for (const auto& pr : probes) {
+            const q err = (q)lt::tri_det(pr[0], pr[1], pr[2]) - qdet(pr[0], pr[1], pr[2]);
+            const double rel = m2 > 0 ? (double)((err < 0 ? -err : err) / m2) : 0.;
+            if (rel > st.det_worst) st.det_worst = rel;
+            if (rel > 4.0e-15) ++st.det_bad;
+        }
+    }
+    // the polynomial against the exact determinant of the computed corners; the bound uses the
+    // coordinate bound of the proof: max(|p| + |dr| |g|, |P|)
+    const double mc = amax(amax(amax(c.xp, c.yp), c.xq), c.yq) +
+                      std::fabs(dr) * amax(amax(amax(amax(c.g0x, c.g0y), amax(c.g1x, c.g1y)), amax(c.g2x, c.g2y)),
+                                           amax(c.g3x, c.g3y));
+    const double mp = mc > amax(px, py) ? mc : amax(px, py);
+    const q mp2 = (q)mp * mp;
+    const double du = __builtin_fma(dr, __builtin_fma(dr, f.c2u, f.c1u), f.c0);
+    const double dl = __builtin_fma(dr, __builtin_fma(dr, f.c2l, f.c1l), f.c0);
+    const q eu = (q)du - qdet(v0, v1, v2), el = (q)dl - qdet(v3, v2, v1);
+    for (q e : {eu, el}) {
+        const double rel = mp2 > 0 ? (double)((e < 0 ? -e : e) / mp2) : 0.;
+        if (rel > st.poly_worst) st.poly_worst = rel;
+        if (rel > 3.2e-14) ++st.poly_bad;
+    }
+    if (f.d_ok) {
+        const q du_x = qdet(v0, v1, v2), dl_x = qdet(v3, v2, v1);
+        const q need = (q)3.2e-11 * mp2;
+        if ((du_x < 0 ? -du_x : du_x) <= need || (dl_x < 0 ? -dl_x : dl_x) <= need) ++st.dok_bad;
+    }
+}
+
+// pmax and the D_ls/D_s range of a one-source sample, as the kernel's per-sample bound forms them
+static void bounds(double px, double py, double dr, double* pmax, double* lo, double* hi)
+{
+    const bool fin = std::fabs(px) <= 1e100 && std::fabs(py) <= 1e100 && std::fabs(dr) <= 1e100;
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    *pmax = fin ? std::fmax(std::fabs(px), std::fabs(py)) : nan;
+    *lo = fin ? dr : nan;
+    *hi = fin ? dr : nan;
+}
+
+int main(int argc, char** argv)
+{
+    const long n = argc > 1 ? std::atol(argv[1]) : 2000000;
+    std::mt19937_64 rng(20240601);
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    std::normal_distribution<double> N(0.0, 1.0);
+    Stats st;
+    const double nan = std::numeric_limits<double>::quiet_NaN(), inf = std::numeric_limits<double>::infinity();
+    for (long t = 0; t < n; ++t) {
+        CellCorners c;
+        const int regime = (int)(t % 6);
+        const double h = std::pow(10.0, -3.0 + 4.0 * U(rng));  // cell size 1e-3 .. 10
+        const double x0 = (U(rng) - 0.5) * 200.0, y0 = (U(rng) - 0.5) * 200.0;
+        c.xp = x0;
+        c.yp = y0;
+        c.xq = c.xp + h;
+        c.yq = c.yp + h * (0.5 + U(rng));
+        double dr = 0.05 + 1.5 * U(rng);
+        const double px_[4] = {c.xp, c.xp, c.xq, c.xq}, py_[4] = {c.yp, c.yq, c.yp, c.yq};
+        double gxv[4], gyv[4];
+        bool lens_far = false;
+        if (regime == 0 || regime == 1) {
+            // smooth lens-like field: affine + curvature, deflections up to ~50"
+            const double a = 2 * U(rng) - 1, b = 2 * U(rng) - 1, cc = 2 * U(rng) - 1, e = 2 * U(rng) - 1;
+            const double kx = 1e-3 * N(rng), ky = 1e-3 * N(rng);
+            const double bx = 20 * N(rng), by = 20 * N(rng);
+            for (int i = 0; i < 4; ++i) {
+                const double X = px_[i] - x0, Y = py_[i] - y0;
+                gxv[i] = bx + a * X + b * Y + kx * X * X + 0.2 * x0;
+                gyv[i] = by + cc * X + e * Y + ky * Y * Y + 0.2 * y0;
+            }
+        } else if (regime == 2) {
+            // mapped corners chosen first: collinear (or nearly) triangles, then g = (p - v) / dr
+            const double vx = (U(rng) - 0.5) * 100, vy = (U(rng) - 0.5) * 100;
+            const double ux = N(rng), uy = N(rng);
+            const double eps = U(rng) < 0.2 ? 0.0 : std::pow(10.0, -17.0 + 10.0 * U(rng));
+            for (int i = 0; i < 4; ++i) {
+                const double s = h * (U(rng) - 0.5) * 3;
+                const double wx = vx + s * ux + eps * N(rng), wy = vy + s * uy + eps * N(rng);
+                gxv[i] = (px_[i] - wx) / dr;
+                gyv[i] = (py_[i] - wy) / dr;
+            }
+        } else if (regime == 3) {
+            // collapsed cell: every corner maps to (nearly) the same point
+            const double vx = (U(rng) - 0.5) * 50, vy = (U(rng) - 0.5) * 50;
+            for (int i = 0; i < 4; ++i) {
+                gxv[i] = (px_[i] - vx) / dr;
+                gyv[i] = (py_[i] - vy) / dr;
+            }
+        } else if (regime == 4) {
+            // wild gradients, magnitudes up to 1e8
+            const double sc = std::pow(10.0, 8.0 * U(rng));
+            for (int i = 0; i < 4; ++i) {
+                gxv[i] = sc * N(rng);
+                gyv[i] = sc * N(rng);
+            }
+        } else {
+            // lens-like with a non-finite value somewhere (gradient, source or dr)
+            for (int i = 0; i < 4; ++i) {
+                gxv[i] = 10 * N(rng) + 0.3 * (px_[i] - x0);
+                gyv[i] = 10 * N(rng) + 0.3 * (py_[i] - y0);
+            }
+            const int where = (int)(U(rng) * 8);
+            const double bad = U(rng) < 0.5 ? nan : (U(rng) < 0.5 ? inf : -inf);
+            if (where < 4) gxv[where] = bad;
+            else if (where < 6) gyv[where - 4] = bad;
+            else if (where == 6) dr = bad;
+        }
+        c.g0x = gxv[0];
+        c.g0y = gyv[0];
+        c.g1x = gxv[1];
+        c.g1y = gyv[1];
+        c.g2x = gxv[2];
+        c.g2y = gyv[2];
+        c.g3x = gxv[3];
+        c.g3y = gyv[3];
+        // the source: on a mapped corner, on an edge midpoint, near the cell, or anywhere
+        const P2 v0 = lt::to_source(c.xp, c.yp, dr, c.g0x, c.g0y);
+        const P2 v1 = lt::to_source(c.xp, c.yq, dr, c.g1x, c.g1y);
+        const P2 v2 = lt::to_source(c.xq, c.yp, dr, c.g2x, c.g2y);
+        const P2 v3 = lt::to_source(c.xq, c.yq, dr, c.g3x, c.g3y);
+        const P2 vs[4] = {v0, v1, v2, v3};
+        double px, py;
+        const double r = U(rng);
+        if (r < 0.15) {
+            const P2 v = vs[(int)(U(rng) * 4)];
+            px = v.x;
+            py = v.y;
+        } else if (r < 0.3) {
+            const P2 a = vs[(int)(U(rng) * 4)], b = vs[(int)(U(rng) * 4)];
+            px = 0.5 * (a.x + b.x);
+            py = 0.5 * (a.y + b.y);
+        } else if (r < 0.7) {
+            const double span = std::fabs(v3.x - v0.x) + std::fabs(v3.y - v0.y) + std::fabs(v1.x - v2.x) + 1e-300;
+            px = v0.x + span * 3 * (U(rng) - 0.3);
+            py = v0.y + span * 3 * (U(rng) - 0.3);
+        } else {
+            px = (U(rng) - 0.5) * 200;
+            py = (U(rng) - 0.5) * 200;
+            lens_far = regime <= 1 && h >= 0.05 && std::hypot(px - v0.x, py - v0.y) > 20 * (h + dr * 2 * h);
+        }
+        if (regime == 5 && (int)(U(rng) * 8) == 7) px = U(rng) < 0.5 ? nan : inf;
+        double pmax, lo, hi;
+        bounds(px, py, dr, &pmax, &lo, &hi);
+        // sometimes looser bounds, as other sources of the sample would give
+        if (U(rng) < 0.3) {
+            pmax *= 1 + 10 * U(rng);
+            lo -= U(rng) * std::fabs(lo);
+            hi += U(rng) * std::fabs(hi);
+        }
+        check(st, c, px, py, dr, pmax, lo, hi, lens_far);
+    }
+    std::printf("trials %ld culled %ld hits %ld unsound %ld\n", st.trials, st.culled, st.hits, st.unsound);
+    std::printf("tri_det error worst %.3g M^2 (bound 4.0e-15, over: %ld); d(dr) error worst %.3g M^2 (bound 3.2e-14, over: %ld)\n",
+                st.det_worst, st.det_bad, st.poly_worst, st.poly_bad);
+    std::printf("cells with d_ok %ld (violations %ld)\n", st.dok, st.dok_bad);
+    std::printf("lens-like far pairs %ld culled %ld (%.4f)\n", st.far_pairs, st.far_culled,
+                st.far_pairs ? (double)st.far_culled / st.far_pairs : 0.0);
+    const bool ok = st.unsound == 0 && st.det_bad == 0 && st.poly_bad == 0 && st.dok_bad == 0 && st.dok > 0 &&
+                    st.hits > 0 &&
+                    st.far_pairs > 0 && st.far_culled >= 0.999 * st.far_pairs;
+    std::printf("%s\n", ok ? "OK" : "FAIL");
+    return ok ? 0 : 1;
+}

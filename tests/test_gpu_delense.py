@@ -88,6 +88,43 @@ def test_delense_no_hits(lt, oracle):
     np.testing.assert_array_equal(loc, [0, 0])
 
 
+def test_delense_degenerate_and_nonfinite(lt, oracle):
+    """Cells the scan's cull must leave to the full reference test: a patch whose corners all map to
+    one source-plane point (zero-area triangles, which the reference's border-inclusive test accepts
+    for any source), sources on a mapped corner and on a cell edge, NaN and infinite gradients, a NaN
+    source and an infinite D_ls/D_s. Found images, their order, counts and status equal the oracle's."""
+    n = 48
+    frame = M.grid_param(-10.0, 10.0, -10.0, 10.0)
+    xs = np.linspace(-10.0, 10.0, n)
+    X, Y = np.meshgrid(xs, xs)
+    r = np.hypot(X, Y) + 0.3
+    gx = 4.0 * X / r + 0.1 * Y
+    gy = 4.0 * Y / r - 0.05 * X
+    dr0, c = 0.8, (1.5, -2.0)
+    gx[30:36, 5:12] = (X[30:36, 5:12] - c[0]) / dr0
+    gy[30:36, 5:12] = (Y[30:36, 5:12] - c[1]) / dr0
+    gx[10, 40] = np.nan
+    gy[12, 20] = np.inf
+    gx[40, 30] = -np.inf
+    gx, gy = gx.ravel(), gy.ravel()
+    src = M.galaxies_array(7)
+    dr = 0.9
+    j = 20 * n + 23  # a grid point away from the patches: the source sits exactly on its mapped corner
+    src["cx"] = [c[0], 0.7, xs[23] - dr * gx[j], 0.5 * ((xs[23] - dr * gx[j]) + (xs[24] - dr * gx[j + 1])),
+                 np.nan, 2.0, -3.0]
+    src["cy"] = [c[1], -0.4, xs[20] - dr * gy[j], 0.5 * ((xs[20] - dr * gy[j]) + (xs[20] - dr * gy[j + 1])),
+                 1.0, 0.5, 3.5]
+    src["dr"] = [dr0, dr0, dr, dr, 0.9, np.inf, 0.7]
+    _case("delense degenerate and non-finite cells")
+    st, pos, loc, numimg = lt.delense_barycenter(7, n, frame, src, gx, gy)
+    ost, opos, oloc, onum = oracle.delense_barycenter(7, n, frame, src, gx, gy)
+    assert st == ost
+    assert numimg == onum
+    np.testing.assert_array_equal(loc, oloc)
+    np.testing.assert_array_equal(pos, opos)
+    assert loc[0] == 20 and loc[2] >= 1
+
+
 def test_chi_computation_golden(lt, oracle):
     z = np.load(os.path.join(GOLDEN, "chi.npz"))
     images = np.ascontiguousarray(z["images"])
