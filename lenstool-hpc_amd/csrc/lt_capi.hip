@@ -558,7 +558,8 @@ struct lt_model_s {
     size_t bytes = 0;
     size_t rec_bytes = 0, rec32_bytes = 0;  // all samples
     void* base = nullptr;                   // device: records [S*H], fp32 records [S*H], runs
-    void* staging = nullptr;                // pinned host image of the two record arrays
+    size_t off_runs = 0;                    // offset of the runs in `base` and in `staging`
+    void* staging = nullptr;                // pinned host image of base (records and runs)
     hipEvent_t staged = nullptr;            // completes when the last upload has read `staging`
     bool staged_pending = false;
 };
@@ -574,6 +575,8 @@ static Use use_from_kind(int use)
     }
 }
 
+// the same run structure (the circular-SIS flag may differ; a batch's run is circular only when it is
+// in every sample)
 static bool same_runs(const std::vector<Run>& a, const std::vector<Run>& b)
 {
     if (a.size() != b.size()) return false;
@@ -598,6 +601,7 @@ static int stage_samples(lt_model_s* m, const lt_potential_soa* lenses, int nsam
     std::vector<Run> runs;
     std::string err;
     char* st = (char*)m->staging;
+    std::vector<Run> batch_runs = m->runs_host;
     for (int s = 0; s < nsamples; ++s) {
         const lt_potential_soa& L = lenses[s];
         for (int i = 0; i < H; ++i)
@@ -607,19 +611,21 @@ static int stage_samples(lt_model_s* m, const lt_potential_soa* lenses, int nsam
         if (rc) return fail(rc, "sample " + std::to_string(s) + ": " + err);
         if (!same_runs(runs, m->runs_host))
             return fail(LT_ERR_INVALID, "sample " + std::to_string(s) + ": run structure (types / dlsds) differs");
+        for (size_t r = 0; r < runs.size(); ++r) batch_runs[r].circ = s ? (batch_runs[r].circ & runs[r].circ) : runs[r].circ;
         if (H) {
             std::memcpy(st + (size_t)s * H * sizeof(HaloRec), recs.data(), (size_t)H * sizeof(HaloRec));
             std::memcpy(st + m->rec_bytes + (size_t)s * H * sizeof(HaloRec32), recs32.data(),
                         (size_t)H * sizeof(HaloRec32));
         }
     }
+    if (!batch_runs.empty()) std::memcpy(st + m->off_runs, batch_runs.data(), batch_runs.size() * sizeof(Run));
     return LT_OK;
 }
 
 // Queue the copy staging -> device on `s`.
 static int upload_staged(lt_model_s* m, hipStream_t s)
 {
-    const size_t n = m->rec_bytes + m->rec32_bytes;
+    const size_t n = m->off_runs + m->runs_host.size() * sizeof(Run);
     if (!n) return LT_OK;
     HIP_TRY(hipMemcpyAsync(m->base, m->staging, n, hipMemcpyHostToDevice, s));
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
@@ -668,9 +674,10 @@ int lt_model_create_batch(lt_model* out, const lt_potential_soa* lenses, int nsa
     m->rec_bytes = (size_t)nsamples * nhalos * sizeof(HaloRec);
     m->rec32_bytes = (size_t)nsamples * nhalos * sizeof(HaloRec32);
     const size_t off_runs = ((m->rec_bytes + m->rec32_bytes + 63) / 64) * 64;
+    m->off_runs = off_runs;
     m->bytes = off_runs + runs.size() * sizeof(Run) + 64;
     hipError_t e = hipMalloc(&m->base, m->bytes);
-    if (e == hipSuccess) e = hipHostMalloc(&m->staging, std::max<size_t>(m->rec_bytes + m->rec32_bytes, 64));
+    if (e == hipSuccess) e = hipHostMalloc(&m->staging, m->bytes);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&m->staged, hipEventDisableTiming);
     if (e != hipSuccess) {
         (void)hipGetLastError();
@@ -686,9 +693,7 @@ int lt_model_create_batch(lt_model* out, const lt_potential_soa* lenses, int nsa
         free_model(m);
         return st;
     }
-    e = hipMemcpy(m->base, m->staging, m->rec_bytes + m->rec32_bytes, hipMemcpyHostToDevice);
-    if (!runs.empty() && e == hipSuccess)
-        e = hipMemcpy(m->dm.runs, runs.data(), runs.size() * sizeof(Run), hipMemcpyHostToDevice);
+    e = hipMemcpy(m->base, m->staging, off_runs + runs.size() * sizeof(Run), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         free_model(m);
         return hip_fail(e, "model upload");

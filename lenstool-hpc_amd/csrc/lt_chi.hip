@@ -225,6 +225,8 @@ int lt_chi_batch_run(lt_chi_batch b, const lt_potential_soa* lenses, double* chi
     std::vector<HaloRec32> recs32;
     std::vector<Run> runs;
     std::string err;
+    // a run uses the circular-SIS terms only when it is circular in every sample
+    for (Run& r : b->runs) r.circ = 1;
     for (int i = 0; i < b->S; ++i) {
         const lt_potential_soa& L = lenses[i];
         for (int h = 0; h < b->H; ++h)
@@ -232,11 +234,15 @@ int lt_chi_batch_run(lt_chi_batch b, const lt_potential_soa* lenses, double* chi
                 return set_error(LT_ERR_INVALID, "sample " + std::to_string(i) + ": halo types differ from the batch's");
         int st = pack_model(&L, b->H, Use::Gradient, recs, recs32, runs, err);
         if (st) return set_error(st, "sample " + std::to_string(i) + ": " + err);
+        if (runs.size() != b->runs.size()) return set_error(LT_ERR_INVALID, "sample " + std::to_string(i) + ": run structure differs");
+        for (size_t r = 0; r < runs.size(); ++r) b->runs[r].circ &= runs[r].circ;
         if (b->H) std::memcpy(&b->staging[(size_t)i * b->H], recs.data(), (size_t)b->H * sizeof(HaloRec));
     }
     if (b->H)
         LT_HIP_TRY(hipMemcpyAsync(b->recs, b->staging.data(), (size_t)b->S * b->H * sizeof(HaloRec),
                                   hipMemcpyHostToDevice, s));
+    if (!b->runs.empty())
+        LT_HIP_TRY(hipMemcpyAsync(b->druns, b->runs.data(), b->runs.size() * sizeof(Run), hipMemcpyHostToDevice, s));
     int st = reset_counters(b, s);
     if (st) return st;
     // gradient grids of every sample (src/chi_CPU_barycenter.cpp:125-137)

@@ -1,6 +1,6 @@
 // lt_device.h -- device-side data layout shared by the kernels and the host packer.
 //
-// A lens model lives in HBM as an array of 256-byte halo records (one per halo, in the SOA order of
+// A lens model lives in HBM as an array of 384-byte halo records (one per halo, in the SOA order of
 // the caller, i.e. grouped by type) plus a short array of runs (type, first halo, count, weight).
 // Every per-halo constant of the closed forms is precomputed on the host once per model, so the
 // per-(halo, pixel) work is only what depends on the pixel. All lanes of a wavefront walk the same
@@ -17,7 +17,8 @@ namespace lt {
 //   8, 81   : PIEMD constants (see below); ampg / amph gradient / Hessian amplitudes; csx .. k5,
 //             ra2c, rb2c, qT .. qs2h the constants of the Hessian's scaled division-free form
 //             (lt_terms64.h)
-//   5       : sa = 1 - eps, sb = 1 + eps, ampg = b0, amph = b0 (1 - eps)(1 + eps)
+//   5       : sa = 1 - eps, sb = 1 + eps, ampg = b0, amph = b0 (1 - eps)(1 + eps); circ = 1 when
+//             eps = 0 (runs of circular SIS form their terms from the lab-frame offsets)
 //   14      : h14a, h14b, h14c = constant Hessian (lab frame)
 struct alignas(64) HaloRec {
     double xh, yh, cs, sn;          // 0..3
@@ -39,7 +40,8 @@ struct alignas(64) HaloRec {
     double k4, k5;                  // 40, 41 1 / cx1^2, 2 sqrt(e) a   (a, b = 1/(1 +- e))
     double pkp, pkq;                // 42, 43 potential: 2 sqrt(e) / (1 - e), 2 sqrt(e) / (1 + e)
     double irc, ircut;              // 44, 45 1 / rcore, 1 / rcut
-    double pad[2];                  // 46, 47
+    double circ;                    // 46     1 for a circular SIS (e = 0)
+    double pad;                     // 47
 };
 static_assert(sizeof(HaloRec) == 384, "halo record is 384 bytes");
 
@@ -48,7 +50,7 @@ struct alignas(32) Run {
     int32_t type;
     int32_t start;
     int32_t count;
-    int32_t pad;
+    int32_t circ;       // 1: every halo of the run is a circular SIS (in every sample of a batch)
     double weight;      // 1 except for the Kmap (dlsds of the run)
     double pad2;
 };
@@ -67,7 +69,8 @@ struct alignas(64) HaloRec32 {
     float hT, hc2, hs2m2, hs2h;         // 13..16  lab-frame amplitudes: trace, cos 2t, -2 sin 2t, sin 2t / 2
     float near2;                        // 17
     float sa, sb;                       // 18, 19  SIS: 1 - e, 1 + e
-    float pad[12];                      // 20..31
+    float circ;                         // 20      1 for a circular SIS
+    float pad[11];                      // 21..31
 };
 static_assert(sizeof(HaloRec32) == 128, "fp32 halo record is 128 bytes");
 

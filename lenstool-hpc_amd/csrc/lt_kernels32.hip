@@ -44,15 +44,15 @@ struct Frame32 {
     float x, y;
 };
 
-// the row part (the same for every pixel of a thread): ty snx, ty csy, ty^2
+// the row part (the same for every pixel of a thread): ty snx, ty csy, ty^2, ty
 struct Row32 {
-    float tys, tyc, ty2;
+    float tys, tyc, ty2, ty;
 };
 
 __device__ __forceinline__ Row32 row32(const HaloRec32& h, double yh, double py)
 {
     const float ty = (float)(py - yh);
-    return Row32{ty * h.snx, ty * h.csy, ty * ty};
+    return Row32{ty * h.snx, ty * h.csy, ty * ty, ty};
 }
 
 // lab-frame x offset of a pixel from the halo centre: the FP64 difference rounded once to FP32
@@ -147,6 +147,21 @@ __device__ __forceinline__ void hess_sis32(const HaloRec32& h, const Frame32& f,
     acc_lab(h, g * (x2 + y2), g * (y2 - x2), -(g * f.x * f.y), fT, fD, fB);
 }
 
+// circular SIS (a run of them): the lab-frame offsets tx, ty give the Hessian directly (hT = b0)
+__device__ __forceinline__ void hess_sis32_circ(const HaloRec32& h, float tx, float ty, float ty2, float& fT, float& fD,
+                                                float& fB)
+{
+    const float x2 = tx * tx;
+    const float iR = rsq32(x2 + ty2);
+    const float ga = h.hT * (iR * iR * iR);
+    fT = __builtin_fmaf(h.hT, iR, fT);
+    fD = __builtin_fmaf(ga, ty2 - x2, fD);
+    fB = __builtin_fmaf(-ga, tx * ty, fB);
+}
+
+// template tag of run32 for a run of circular SIS (not a reference type code)
+constexpr int TYPE_SIS_CIRC = -5;
+
 // One run of halos of one type: FP32 terms summed per chunk of CHUNK halos, FP64 terms (near
 // lanes) straight into the FP64 run sums.
 template <int TYPE, int PX>
@@ -170,7 +185,10 @@ __device__ __forceinline__ void run32(const HaloRec* __restrict__ h64, const Hal
                     const HaloRec& hd = h64[i];
                     if constexpr (TYPE == TYPE_DPIE) hess_piemd<true>(hd, px[k], py, sT[k], sD[k], sB[k]);
                     else if constexpr (TYPE == TYPE_PIEMD) hess_piemd<false>(hd, px[k], py, sT[k], sD[k], sB[k]);
+                    else if constexpr (TYPE == TYPE_SIS_CIRC) hess_sis_circ(hd, px[k], py, sT[k], sD[k], sB[k]);
                     else hess_sis(hd, px[k], py, sT[k], sD[k], sB[k]);
+                } else if constexpr (TYPE == TYPE_SIS_CIRC) {
+                    hess_sis32_circ(h, tx, rw.ty, rw.ty2, fT[k], fD[k], fB[k]);
                 } else {
                     const Frame32 f = frame32(h, rw, tx);
                     if constexpr (TYPE == TYPE_DPIE) hess_piemd32<true>(h, f, fT[k], fD[k], fB[k]);
@@ -202,6 +220,7 @@ __device__ __forceinline__ void hessian_fp64_pixel(const HaloRec* __restrict__ h
             const HaloRec& h = h64[i];
             if (run.type == TYPE_DPIE) hess_piemd<true>(h, px, py, sT, sD, sB);
             else if (run.type == TYPE_PIEMD) hess_piemd<false>(h, px, py, sT, sD, sB);
+            else if (run.type == TYPE_SIS && run.circ) hess_sis_circ(h, px, py, sT, sD, sB);
             else if (run.type == TYPE_SIS) hess_sis(h, px, py, sT, sD, sB);
             else {
                 sT += h.h14a + h.h14c;
@@ -260,6 +279,8 @@ __global__ __launch_bounds__(WAVE* ROWS) void k_hessian32(const HaloRec* __restr
             run32<TYPE_DPIE, PX>(h64, h32, run.start, end, px, py, sT, sD, sB);
         } else if (run.type == TYPE_PIEMD) {
             run32<TYPE_PIEMD, PX>(h64, h32, run.start, end, px, py, sT, sD, sB);
+        } else if (run.circ) {
+            run32<TYPE_SIS_CIRC, PX>(h64, h32, run.start, end, px, py, sT, sD, sB);
         } else {
             run32<TYPE_SIS, PX>(h64, h32, run.start, end, px, py, sT, sD, sB);
         }

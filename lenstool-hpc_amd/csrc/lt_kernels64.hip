@@ -54,10 +54,18 @@ __global__ __launch_bounds__(WAVE* ROWS) void k_hessian64(const HaloRec* __restr
             }
             break;
         case TYPE_SIS:
-            for (int i = run.start; i < end; ++i) {
-                const HaloRec& h = halos[i];
+            if (run.circ) {
+                for (int i = run.start; i < end; ++i) {
+                    const HaloRec& h = halos[i];
 #pragma unroll
-                for (int k = 0; k < PX; ++k) hess_sis(h, px[k], py, sT[k], sD[k], sB[k]);
+                    for (int k = 0; k < PX; ++k) hess_sis_circ(h, px[k], py, sT[k], sD[k], sB[k]);
+                }
+            } else {
+                for (int i = run.start; i < end; ++i) {
+                    const HaloRec& h = halos[i];
+#pragma unroll
+                    for (int k = 0; k < PX; ++k) hess_sis(h, px[k], py, sT[k], sD[k], sB[k]);
+                }
             }
             break;
         default:  // TYPE_SHEAR: constant lab-frame Hessian
@@ -125,10 +133,18 @@ __global__ __launch_bounds__(WAVE* ROWS) void k_gradient64(const HaloRec* __rest
             }
             break;
         default:  // TYPE_SIS
-            for (int i = run.start; i < end; ++i) {
-                const HaloRec& h = halos[i];
+            if (run.circ) {
+                for (int i = run.start; i < end; ++i) {
+                    const HaloRec& h = halos[i];
 #pragma unroll
-                for (int k = 0; k < PX; ++k) grad_sis(h, px[k], py, sx[k], sy[k]);
+                    for (int k = 0; k < PX; ++k) grad_sis_circ(h, px[k], py, sx[k], sy[k]);
+                }
+            } else {
+                for (int i = run.start; i < end; ++i) {
+                    const HaloRec& h = halos[i];
+#pragma unroll
+                    for (int k = 0; k < PX; ++k) grad_sis(h, px[k], py, sx[k], sy[k]);
+                }
             }
             break;
         }
@@ -197,7 +213,10 @@ __global__ __launch_bounds__(256) void k_points_gradient64(const HaloRec* __rest
             for (int h = run.start; h < end; ++h) grad_piemd<false>(halos[h], x, y, sx, sy);
             break;
         default:
-            for (int h = run.start; h < end; ++h) grad_sis(halos[h], x, y, sx, sy);
+            if (run.circ)
+                for (int h = run.start; h < end; ++h) grad_sis_circ(halos[h], x, y, sx, sy);
+            else
+                for (int h = run.start; h < end; ++h) grad_sis(halos[h], x, y, sx, sy);
             break;
         }
         tx += sx;

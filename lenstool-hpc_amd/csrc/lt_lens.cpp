@@ -78,6 +78,7 @@ int pack_model(const lt_potential_soa* L, int n, Use use, std::vector<HaloRec>& 
             h.sb = 1.0 + eps;
             h.ampg = b0;
             h.amph = b0 * h.sa * h.sb;
+            h.circ = (eps == 0.0) ? 1.0 : 0.0;
         } else if (piemd) {
             if (!L->rcore || (t == TYPE_DPIE && !L->rcut)) {
                 err = "lens model: rcore / rcut is NULL";
@@ -173,14 +174,15 @@ int pack_model(const lt_potential_soa* L, int n, Use use, std::vector<HaloRec>& 
             const double R = kFp32NearScalePiemd * std::fabs(b0 * t05);
             f.near2 = (float)std::fmin(R * R, 3.0e38);
         } else if (t == TYPE_SIS) {
-            f.csx = f.csy = (float)cs;
-            f.snx = f.sny = (float)sn;
             f.sa = (float)h.sa;
             f.sb = (float)h.sb;
             f.hT = (float)h.amph;
+            f.csx = f.csy = (float)cs;
+            f.snx = f.sny = (float)sn;
             f.hc2 = (float)h.hc2;
             f.hs2m2 = (float)(-2.0 * h.hs2);
             f.hs2h = (float)(0.5 * h.hs2);
+            f.circ = (float)h.circ;
             const double R = kFp32NearScaleSis * std::fabs(b0);
             f.near2 = (float)(R * R);
         } else {
@@ -200,6 +202,8 @@ int pack_model(const lt_potential_soa* L, int n, Use use, std::vector<HaloRec>& 
         r.start = s;
         r.count = c;
         r.weight = w;
+        r.circ = (t == TYPE_SIS) ? 1 : 0;
+        for (int i = s; i < s + c && r.circ; ++i) r.circ = recs[i].circ != 0.0;
         runs.push_back(r);
         s += c;
     }

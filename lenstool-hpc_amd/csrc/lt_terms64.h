@@ -138,6 +138,20 @@ __device__ __forceinline__ void hess_sis(const HaloRec& h, double px, double py,
     aB = __builtin_fma(g, __builtin_fma(0.5 * h.hs2, dyx, -(h.hc2 * xy)), aB);
 }
 
+// Circular SIS (e = 0): the Hessian b0/r^3 (y^2, -xy, x^2) holds in any frame, so it is formed from
+// the lab-frame offsets, with no rotation to the halo frame and back (T = b0 / r).
+__device__ __forceinline__ void hess_sis_circ(const HaloRec& h, double px, double py, double& aT, double& aD,
+                                              double& aB)
+{
+    const double tx = px - h.xh, ty = py - h.yh;
+    const double x2 = tx * tx, y2 = ty * ty;
+    const double iR = rsq64_n2(x2 + y2);
+    const double ga = h.amph * (iR * iR * iR);
+    aT = __builtin_fma(h.amph, iR, aT);
+    aD = __builtin_fma(ga, y2 - x2, aD);
+    aB = __builtin_fma(-ga, tx * ty, aB);
+}
+
 // ------------------------------------------------------------------------------------------------
 // gradient terms, accumulated in the lab frame
 // ------------------------------------------------------------------------------------------------
@@ -205,6 +219,15 @@ __device__ __forceinline__ void grad_sis(const HaloRec& h, double px, double py,
     double ry = h.sb * f.y * s;
     gx += __builtin_fma(rx, h.cs, -(ry * h.sn));
     gy += __builtin_fma(ry, h.cs, rx * h.sn);
+}
+
+// Circular SIS: b0 (x, y) / r in any frame, formed from the lab-frame offsets.
+__device__ __forceinline__ void grad_sis_circ(const HaloRec& h, double px, double py, double& gx, double& gy)
+{
+    const double tx = px - h.xh, ty = py - h.yh;
+    const double s = h.ampg * rsq64(__builtin_fma(tx, tx, ty * ty));
+    gx = __builtin_fma(s, tx, gx);
+    gy = __builtin_fma(s, ty, gy);
 }
 
 // ------------------------------------------------------------------------------------------------
