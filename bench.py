@@ -11,7 +11,9 @@ the stream the kernel runs on.
 
 The headline value is the FP64 map (--precision fp64, default); the same steps are then timed in
 the other precision and reported under "fp32c" (or "fp64"), with the FP32-compensated map's maximum
-absolute error against the FP64 map (det A, kappa, gamma1, gamma2) measured after the timed region.
+absolute error against the FP64 map (det A, kappa, gamma1, gamma2) and its relative error on kappa
+and gamma where |det A| >= 1e-3, measured after the timed region; then FP64 gradient maps of the
+same lens and grid ("gradient_fp64").
 Prints one JSON line. Runs on one GPU only (--gpus > 1 exits with a message).
 """
 import argparse
@@ -42,16 +44,16 @@ def percentile(v, q):
     return float(np.percentile(np.asarray(v), q))
 
 
-def run_maps(model, outs, grid, r, precision, steps, warmup, stream, px=0):
+def run_maps(model, outs, grid, r, precision, steps, warmup, stream, px=0, kind=lthip.MAP_FIELDS):
     ptrs = [o.ptr for o in outs]
     for _ in range(warmup):
-        model.map(lthip.MAP_FIELDS, 0, ptrs, grid, r=r, precision=precision, stream=stream, px=px)
+        model.map(kind, 0, ptrs, grid, r=r, precision=precision, stream=stream, px=px)
     lthip.sync(stream)
     ev = [lthip.Event() for _ in range(steps + 1)]
     t0 = time.perf_counter()
     ev[0].record(stream)
     for k in range(steps):
-        model.map(lthip.MAP_FIELDS, 0, ptrs, grid, r=r, precision=precision, stream=stream, px=px)
+        model.map(kind, 0, ptrs, grid, r=r, precision=precision, stream=stream, px=px)
         ev[k + 1].record(stream)
     ev[-1].synchronize()
     wall = time.perf_counter() - t0
@@ -155,11 +157,36 @@ def bench_bayes(a):
             "fp64": res["fp64"], "fp32c": res["fp32c"]}
 
 
+def refine_images(lens, pos, sx, sy, dr, iters=8, h=1e-5):
+    """Newton iteration on theta - dr alpha(theta) = s for image positions pos [k, 2] of one source
+    (s = (sx, sy)), with the library's FP64 point gradients and a central-difference Jacobian; an
+    image whose residual does not shrink keeps its starting position."""
+    th = np.array(pos, dtype=np.float64)
+    def resid(t):
+        gx, gy = lthip.point_gradients(lens, t[:, 0], t[:, 1])
+        return np.stack([t[:, 0] - dr * gx - sx, t[:, 1] - dr * gy - sy], axis=1)
+    r0 = np.hypot(*resid(th).T)
+    for _ in range(iters):
+        F = resid(th)
+        J = np.empty((len(th), 2, 2))
+        for j, e in enumerate(((h, 0.0), (0.0, h))):
+            d = (resid(th + np.array(e)) - resid(th - np.array(e))) / (2 * h)
+            J[:, :, j] = d
+        th = th - np.linalg.solve(J, F[:, :, None])[:, :, 0]
+    r1 = np.hypot(*resid(th).T)
+    keep = ~(r1 < r0) | ~np.isfinite(r1) | ~(np.hypot(*(th - np.asarray(pos)).T) < 1.0)
+    th[keep] = np.asarray(pos)[keep]
+    return th
+
+
 def bench_chi(a):
     """Delensing + chi^2 batched over lens samples: the 1 dPIE + 10 SIS lens with 40 families of 3
-    images (sources uniform in [-20, 20]^2 arcsec, z_s ~ U[1, 6]; images = barycentres of this
-    library's own delensing of the true model on a 2048^2 grid + N(0, 0.1") noise), delensing grid
-    512^2, samples perturbed as for the Bayes maps. A step = one ChiBatch run over all samples."""
+    images (sources uniform in [-20, 20]^2 arcsec, z_s ~ U[1, 6]; images = this library's own
+    delensing of the true model on a 2048^2 grid, refined by Newton iteration on
+    theta - dr alpha(theta) = s, + N(0, 0.1") noise), delensing grid 512^2, samples perturbed as for
+    the Bayes maps. A step = one ChiBatch run over all samples. The CPU baseline is the oracle's
+    chi^2 path (its restatement of the reference's gradient grid, delensing and chi_computation) on a
+    few of the same samples."""
     rng = np.random.default_rng(4)
     lens = M.config1_lens(seed=1)
     cosmo = M.flat_lcdm(70.0, 0.3)
@@ -176,7 +203,8 @@ def bench_chi(a):
         _, pos, loc, _ = lthip.delense_barycenter(16, fine.nx, frame, src, gx.ravel(), gy.ravel())
         for s in range(16):
             if loc[s] >= 3 and len(fams) < 40:
-                fams.append((pos[s, :3].copy(), src["dr"][s], zs[s]))
+                th = refine_images(lens, pos[s, :3], src["cx"][s], src["cy"][s], src["dr"][s])
+                fams.append((th, src["dr"][s], zs[s]))
     nimages = np.full(40, 3, dtype=np.int32)
     images = M.galaxies_array(120)
     for f, (p, dr, z) in enumerate(fams):
@@ -195,6 +223,19 @@ def bench_chi(a):
         chi, found, status = b.run(samples)
     wall = time.perf_counter() - t0
     b.close()
+    cpu = None
+    if not a.no_cpu_baseline:
+        import pyoracle as O
+        threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+        O.set_threads(threads)
+        n, t1 = 0, time.perf_counter()
+        while n < S and (n < 4 or time.perf_counter() - t1 < 10.0):
+            O.chi_barycentersource(samples[n], 512, frame, nimages, images)
+            n += 1
+        dt = time.perf_counter() - t1
+        cpu = {"value": n / dt, "unit": "samples/s", "cores": threads, "kind": "port",
+               "sample": f"the oracle's chi^2 path on the first {n} of the same samples, {dt:.1f} s, "
+                         f"OMP threads = {threads}"}
     return {"metric": "chi^2 samples/s (gradient grid + barycentres + delensing + chi^2 per sample)",
             "value": a.steps * S / wall, "unit": "samples/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
             "ms_per_step": wall / a.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -203,7 +244,7 @@ def bench_chi(a):
                        "samples": S},
             "gradient_halo_px_per_s": a.steps * S * 512 * 512 * lens.n / wall,
             "chi_p50": float(np.median(chi)), "found_mean": float(found.mean()),
-            "overflow_samples": int((status != 0).sum())}
+            "overflow_samples": int((status != 0).sum()), "cpu_baseline": cpu}
 
 
 def bench_serve(a):
@@ -375,12 +416,29 @@ def main():
         second = [o.download() for o in outs[:4]]
         f64, f32 = (first, second) if prec == lthip.PREC_FP64 else (second, first)
         errs = {n: float(np.abs(x - y).max()) for n, x, y in zip(("kappa", "gamma1", "gamma2", "detA"), f32, f64)}
+        # relative errors of kappa and gamma away from the critical lines (|det A| >= 1e-3)
+        away = np.abs(f64[3]) >= 1e-3
+        g64, g32 = np.hypot(f64[1], f64[2]), np.hypot(f32[1], f32[2])
+        mk, mg = away & (f64[0] != 0), away & (g64 != 0)
+        rel = {"kappa": float((np.abs(f32[0] - f64[0])[mk] / np.abs(f64[0][mk])).max()),
+               "gamma": float((np.abs(g32 - g64)[mg] / g64[mg]).max())}
         name = "fp32c" if other == lthip.PREC_FP32C else "fp64"
         v2 = a.steps * halo_px / (total2 * 1e-3)
         res[name] = {"value": v2, "unit": "halo·px/s", "ms_per_step": total2 / a.steps,
                      "latency_ms": {"p50": percentile(per2, 50), "p99": percentile(per2, 99)},
                      "fp32c_over_fp64": (v2 / value) if name == "fp32c" else (value / v2),
-                     "fp32c_max_abs_err_vs_fp64": errs}
+                     "fp32c_max_abs_err_vs_fp64": errs,
+                     "fp32c_max_rel_err_vs_fp64_where_abs_detA_ge_1e-3": rel}
+        # gradient maps of the same lens and grid (FP64; FP32C gradient requests use the FP64 kernel)
+        gm = lthip.Model(lens, lthip.MAP_GRADIENT)
+        gsteps = min(a.steps, 10)
+        perg, totalg, _ = run_maps(gm, outs[:2], grid, 1.0, lthip.PREC_FP64, gsteps, 1, stream, 0, lthip.MAP_GRADIENT)
+        gm.close()
+        res["gradient_fp64"] = {"value": gsteps * halo_px / (totalg * 1e-3), "unit": "halo·px/s",
+                                "ms_per_step": totalg / gsteps, "steps": gsteps,
+                                "latency_ms": {"p50": percentile(perg, 50), "p99": percentile(perg, 99)},
+                                "algorithmic_TFLOP/s": lens.halo_flops("gradient") * grid.npix /
+                                (totalg / gsteps * 1e-3) / 1e12}
     if a.copy_bytes > 0:
         src = lthip.DeviceArray(a.copy_bytes, np.uint8)
         dst = lthip.DeviceArray(a.copy_bytes, np.uint8)
