@@ -178,8 +178,8 @@ int lt_graph_destroy(lt_graph graph);
  * The reference has no counterpart (its map drivers, utils/exec/main.cpp:397-545, run one
  * synchronous map after another). A server keeps lens models resident on the calling thread's GPU,
  * runs requests on nslots streams and admits queued requests as slots and HBM (hbm_budget_bytes,
- * counting the output planes of admitted requests and its cache of freed planes) become free; work
- * of different requests overlaps on the device. Results stay in device memory until fetched. */
+ * reserved at creation; the output planes of admitted requests are carved out of it) become free;
+ * work of different requests overlaps on the device. Results stay in device memory until fetched. */
 typedef struct lt_server_s* lt_server;
 typedef struct lt_request {
     int model;            /* id from lt_server_add_model                                          */
@@ -199,6 +199,8 @@ typedef struct lt_server_stats {
     double elapsed_ms;                  /* since creation or the last reset                       */
     long long bytes_in_use, bytes_peak;
 } lt_server_stats;
+/* nslots concurrent requests; hbm_budget_bytes of device memory are reserved at creation (the
+ * result planes of admitted requests are carved out of it, so serving allocates nothing) */
 int lt_server_create(lt_server* out, int nslots, long long hbm_budget_bytes);
 /* Pack the model for every map kind it supports (gradient, Hessian, Kmap when dlsds is given,
  * potential); *model_id indexes it in requests. */

@@ -7,11 +7,12 @@
 // copying back synchronously. Here one dispatcher thread owns all launches:
 //   submit()  -> queue (FIFO, with bounded bypass so that small requests can pass a large one
 //                that does not fit yet)
-//   dispatch  -> admit while a slot is free and the request's planes fit in the budget, allocate
-//                the planes from a size-keyed cache, launch lt_map_device on the slot's stream,
-//                record an event
+//   dispatch  -> admit while a slot is free and the request's planes fit in the budget, carve
+//                them out of one device arena reserved at creation (so that serving never calls
+//                hipMalloc or hipFree, which can stall the device), launch lt_map_device on the
+//                slot's stream, record an event
 //   poll      -> events that completed mark their tickets done and free their slots
-//   fetch()   -> caller thread: wait, copy planes to host on its own stream, release memory.
+//   fetch()   -> caller thread: wait, copy planes to host on a pooled stream, release memory.
 // Every pixel is computed by the same code whatever else runs, so a request's result is the same
 // alone and in a mix.
 #include <hip/hip_runtime.h>
@@ -47,6 +48,7 @@ struct Ticket {
     std::string err;
     int nplanes = 0;
     size_t plane_bytes = 0;
+    size_t block_off = 0, block_len = 0;  // the planes' block in the arena (block_len 0: none)
     double* planes[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     int slot = -1;
     hipEvent_t done_ev = nullptr;
@@ -106,6 +108,59 @@ double percentile(std::vector<double> v, double q)
     return i + 1 < v.size() ? v[i] * (1 - f) + v[i + 1] * f : v[i];
 }
 
+constexpr size_t kAlign = 256;
+
+size_t align_up(size_t n) { return (n + kAlign - 1) / kAlign * kAlign; }
+
+// First-fit allocator over one device block: free ranges by offset, coalesced on release.
+struct Arena {
+    char* base = nullptr;
+    size_t size = 0;
+    std::map<size_t, size_t> free_;  // offset -> length
+
+    void reset(char* b, size_t n)
+    {
+        base = b;
+        size = n;
+        free_.clear();
+        if (n) free_.emplace(0, n);
+    }
+    size_t largest() const
+    {
+        size_t m = 0;
+        for (const auto& f : free_) m = std::max(m, f.second);
+        return m;
+    }
+    bool alloc(size_t need, size_t* off)
+    {
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < need) continue;
+            *off = it->first;
+            const size_t rest = it->second - need, at = it->first + need;
+            free_.erase(it);
+            if (rest) free_.emplace(at, rest);
+            return true;
+        }
+        return false;
+    }
+    void release(size_t off, size_t len)
+    {
+        auto it = free_.emplace(off, len).first;
+        auto next = std::next(it);
+        if (next != free_.end() && it->first + it->second == next->first) {
+            it->second += next->second;
+            free_.erase(next);
+        }
+        if (it != free_.begin()) {
+            auto prev = std::prev(it);
+            if (prev->first + prev->second == it->first) {
+                prev->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+};
+
 }  // namespace
 
 struct lt_server_s {
@@ -125,9 +180,12 @@ struct lt_server_s {
     std::unordered_map<long long, std::unique_ptr<Ticket>> tickets;
     std::vector<long long> running;
 
-    // device memory: in use by tickets + cached free blocks, both counted against the budget
-    long long bytes_used = 0, bytes_cached = 0, bytes_peak = 0;
-    std::multimap<size_t, void*> cache;
+    // device memory: one arena of `budget` bytes; bytes_used = blocks held by tickets
+    Arena arena;
+    long long bytes_used = 0, bytes_peak = 0;
+    // streams for the result copies of fetch() (callers may fetch from several threads)
+    std::mutex fetch_mu;
+    std::vector<hipStream_t> fetch_streams;
 
     // statistics over completed requests since the last reset
     std::vector<double> lat_done, lat_fetch;
@@ -137,47 +195,33 @@ struct lt_server_s {
 
     std::thread worker;
 
-    // ---- memory (called with mu held, from the dispatcher) ----------------------------------------
-    void* take(size_t bytes)
+    // ---- memory (called with mu held) -------------------------------------------------------------
+    static size_t block_bytes(const Ticket& t) { return (size_t)t.nplanes * align_up(t.plane_bytes); }
+
+    bool take(Ticket& t)
     {
-        auto it = cache.find(bytes);
-        if (it != cache.end()) {
-            void* p = it->second;
-            cache.erase(it);
-            bytes_cached -= (long long)bytes;
-            bytes_used += (long long)bytes;
-            bytes_peak = std::max(bytes_peak, bytes_used + bytes_cached);
-            return p;
-        }
-        // make room: drop cached blocks of other sizes
-        while (!cache.empty() && bytes_used + bytes_cached + (long long)bytes > budget) {
-            auto victim = cache.begin();
-            (void)hipFree(victim->second);
-            bytes_cached -= (long long)victim->first;
-            cache.erase(victim);
-        }
-        void* p = nullptr;
-        if (hipMalloc(&p, bytes) != hipSuccess) {
-            (void)hipGetLastError();
-            return nullptr;
-        }
-        bytes_used += (long long)bytes;
-        bytes_peak = std::max(bytes_peak, bytes_used + bytes_cached);
-        return p;
+        const size_t need = block_bytes(t);
+        size_t off;
+        if (!arena.alloc(need, &off)) return false;
+        t.block_off = off;
+        t.block_len = need;
+        for (int k = 0; k < t.nplanes; ++k)
+            t.planes[k] = (double*)(arena.base + off + (size_t)k * align_up(t.plane_bytes));
+        bytes_used += (long long)need;
+        bytes_peak = std::max(bytes_peak, bytes_used);
+        return true;
     }
 
-    void give_back(void* p, size_t bytes)
+    void give_back(Ticket& t)
     {
-        bytes_used -= (long long)bytes;
-        bytes_cached += (long long)bytes;
-        cache.emplace(bytes, p);
+        if (!t.block_len) return;
+        arena.release(t.block_off, t.block_len);
+        bytes_used -= (long long)t.block_len;
+        t.block_len = 0;
+        for (int k = 0; k < 5; ++k) t.planes[k] = nullptr;
     }
 
-    bool fits(const Ticket& t) const
-    {
-        // cached blocks can be freed to make room
-        return bytes_used + (long long)(t.plane_bytes * t.nplanes) <= budget;
-    }
+    bool fits(const Ticket& t) const { return arena.largest() >= block_bytes(t); }
 
     // ---- dispatcher -------------------------------------------------------------------------------
     void fail_ticket(Ticket& t, int st, const std::string& msg)
@@ -191,14 +235,9 @@ struct lt_server_s {
 
     void launch(Ticket& t, int slot)
     {
-        for (int k = 0; k < t.nplanes; ++k) {
-            t.planes[k] = (double*)take(t.plane_bytes);
-            if (!t.planes[k]) {
-                for (int j = 0; j < k; ++j) give_back(t.planes[j], t.plane_bytes);
-                for (int j = 0; j < 5; ++j) t.planes[j] = nullptr;
-                fail_ticket(t, LT_ERR_NOMEM, "device allocation of the map planes failed");
-                return;
-            }
+        if (!take(t)) {
+            fail_ticket(t, LT_ERR_NOMEM, "no room for the map planes in the server's arena");
+            return;
         }
         ServedModel& m = *models[t.req.model];
         const int ui = use_index(t.req.kind);
@@ -210,8 +249,7 @@ struct lt_server_s {
                                t.planes[3], t.planes[4], t.req.r, t.req.mult, &g, t.req.precision, 0, slots[slot]);
         if (st == LT_OK && hipEventRecord(t.done_ev, slots[slot]) != hipSuccess) st = LT_ERR_HIP;
         if (st != LT_OK) {
-            for (int k = 0; k < t.nplanes; ++k) give_back(t.planes[k], t.plane_bytes);
-            for (int k = 0; k < 5; ++k) t.planes[k] = nullptr;
+            give_back(t);
             fail_ticket(t, st, lt_last_error());
             return;
         }
@@ -314,6 +352,14 @@ int lt_server_create(lt_server* out, int nslots, long long hbm_budget_bytes)
         s->slots.push_back(h);
     }
     s->slot_busy.assign(nslots, false);
+    void* arena = nullptr;
+    if (hipMalloc(&arena, (size_t)hbm_budget_bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        for (auto x : s->slots) (void)hipStreamDestroy(x);
+        return set_error(LT_ERR_NOMEM, "could not reserve the HBM budget (" + std::to_string(hbm_budget_bytes) +
+                                           " bytes) for the server's arena");
+    }
+    s->arena.reset((char*)arena, (size_t)hbm_budget_bytes);
     lt_server_s* raw = s.release();
     raw->worker = std::thread([raw] { raw->run(); });
     *out = raw;
@@ -361,7 +407,7 @@ int lt_server_submit(lt_server s, const lt_request* req, long long* ticket)
     t->req = *req;
     t->nplanes = planes_of(req->kind);
     t->plane_bytes = (size_t)req->grid.nx * (size_t)req->grid.ny * sizeof(double);
-    if ((long long)(t->plane_bytes * t->nplanes) > s->budget)
+    if ((long long)lt_server_s::block_bytes(*t) > s->budget)
         return set_error(LT_ERR_CAPACITY, "request needs more device memory than the server's budget");
     t->halo_px = (double)m.nhalos * req->grid.nx * req->grid.ny;
     if (hipEventCreateWithFlags(&t->done_ev, hipEventDisableTiming) != hipSuccess)
@@ -408,18 +454,28 @@ int lt_server_fetch(lt_server s, long long ticket, double* out0, double* out1, d
     if (st == LT_OK) {
         (void)hipSetDevice(s->device);
         hipStream_t fs = nullptr;
-        hipError_t e = hipStreamCreateWithFlags(&fs, hipStreamNonBlocking);
+        hipError_t e = hipSuccess;
+        {
+            std::lock_guard<std::mutex> fl(s->fetch_mu);
+            if (!s->fetch_streams.empty()) {
+                fs = s->fetch_streams.back();
+                s->fetch_streams.pop_back();
+            }
+        }
+        if (!fs) e = hipStreamCreateWithFlags(&fs, hipStreamNonBlocking);
         for (int k = 0; k < t->nplanes && e == hipSuccess; ++k)
             if (outs[k]) e = hipMemcpyAsync(outs[k], t->planes[k], t->plane_bytes, hipMemcpyDeviceToHost, fs);
         if (e == hipSuccess) e = hipStreamSynchronize(fs);
-        if (fs) (void)hipStreamDestroy(fs);
+        if (fs) {
+            std::lock_guard<std::mutex> fl(s->fetch_mu);
+            s->fetch_streams.push_back(fs);
+        }
         if (e != hipSuccess) cst = hip_error(e, "result copy");
     }
     t->t_fetched = Clock::now();
     {
         std::lock_guard<std::mutex> lk(s->mu);
-        for (int k = 0; k < t->nplanes; ++k)
-            if (t->planes[k]) s->give_back(t->planes[k], t->plane_bytes);
+        s->give_back(*t);
         if (st == LT_OK) s->lat_fetch.push_back(ms_between(t->t_submit, t->t_fetched));
     }
     if (times) {
@@ -461,7 +517,7 @@ int lt_server_reset_stats(lt_server s)
     s->lat_fetch.clear();
     s->n_submitted = s->n_completed = s->n_failed = 0;
     s->halo_px_done = 0;
-    s->bytes_peak = s->bytes_used + s->bytes_cached;
+    s->bytes_peak = s->bytes_used;
     s->stats_t0 = Clock::now();
     return LT_OK;
 }
@@ -479,11 +535,10 @@ int lt_server_destroy(lt_server s)
     for (auto h : s->slots) (void)hipStreamSynchronize(h);
     for (auto& kv : s->tickets) {
         Ticket& t = *kv.second;
-        for (int k = 0; k < t.nplanes; ++k)
-            if (t.planes[k]) (void)hipFree(t.planes[k]);
         if (t.done_ev) (void)hipEventDestroy(t.done_ev);
     }
-    for (auto& kv : s->cache) (void)hipFree(kv.second);
+    if (s->arena.base) (void)hipFree(s->arena.base);
+    for (auto h : s->fetch_streams) (void)hipStreamDestroy(h);
     for (auto& m : s->models)
         for (auto h : m->use_model)
             if (h) lt_model_destroy(h);
