@@ -12,8 +12,8 @@ the stream the kernel runs on.
 The headline value is the FP64 map (--precision fp64, default); the same steps are then timed in
 the other precision and reported under "fp32c" (or "fp64"), with the FP32-compensated map's maximum
 absolute error against the FP64 map (det A, kappa, gamma1, gamma2) and its relative error on kappa
-and gamma where |det A| >= 1e-3, measured after the timed region; then FP64 gradient maps of the
-same lens and grid ("gradient_fp64").
+and gamma where |det A| >= 1e-3, measured after the timed region; then gradient maps of the same
+lens and grid in both precisions ("gradient_fp64", "gradient_fp32c" with its error against FP64).
 Prints one JSON line. Runs on one GPU only (--gpus > 1 exits with a message).
 """
 import argparse
@@ -432,13 +432,18 @@ def main():
         # gradient maps of the same lens and grid (FP64; FP32C gradient requests use the FP64 kernel)
         gm = lthip.Model(lens, lthip.MAP_GRADIENT)
         gsteps = min(a.steps, 10)
-        perg, totalg, _ = run_maps(gm, outs[:2], grid, 1.0, lthip.PREC_FP64, gsteps, 1, stream, 0, lthip.MAP_GRADIENT)
+        grads = {}
+        for gname, gprec in (("fp64", lthip.PREC_FP64), ("fp32c", lthip.PREC_FP32C)):
+            perg, totalg, _ = run_maps(gm, outs[:2], grid, 1.0, gprec, gsteps, 1, stream, 0, lthip.MAP_GRADIENT)
+            grads[gname] = [o.download() for o in outs[:2]]
+            res["gradient_" + gname] = {
+                "value": gsteps * halo_px / (totalg * 1e-3), "unit": "halo·px/s", "ms_per_step": totalg / gsteps,
+                "steps": gsteps, "latency_ms": {"p50": percentile(perg, 50), "p99": percentile(perg, 99)},
+                "algorithmic_TFLOP/s": lens.halo_flops("gradient") * grid.npix / (totalg / gsteps * 1e-3) / 1e12}
         gm.close()
-        res["gradient_fp64"] = {"value": gsteps * halo_px / (totalg * 1e-3), "unit": "halo·px/s",
-                                "ms_per_step": totalg / gsteps, "steps": gsteps,
-                                "latency_ms": {"p50": percentile(perg, 50), "p99": percentile(perg, 99)},
-                                "algorithmic_TFLOP/s": lens.halo_flops("gradient") * grid.npix /
-                                (totalg / gsteps * 1e-3) / 1e12}
+        res["gradient_fp32c"]["max_abs_err_vs_fp64_arcsec"] = float(
+            np.hypot(grads["fp32c"][0] - grads["fp64"][0], grads["fp32c"][1] - grads["fp64"][1]).max())
+        res["gradient_fp32c"]["fp32c_over_fp64"] = res["gradient_fp32c"]["value"] / res["gradient_fp64"]["value"]
     if a.copy_bytes > 0:
         src = lthip.DeviceArray(a.copy_bytes, np.uint8)
         dst = lthip.DeviceArray(a.copy_bytes, np.uint8)

@@ -15,7 +15,9 @@
  *  2. Resident models and device-pointer entry points: a lens model is packed and uploaded once
  *     (lt_model_create) and maps are computed asynchronously on a caller stream into device memory.
  *
- * precision: LT_PREC_FP64 or LT_PREC_FP32C (lt_status.h).
+ * precision: LT_PREC_FP64 or LT_PREC_FP32C (lt_status.h). FP32C (FP32 halo terms with FP64 offsets,
+ * FP64 terms next to the cores and FP64 chunk sums) applies to the Hessian-derived maps and to the
+ * gradient / displacement maps; potential maps are FP64 only.
  */
 #ifndef LENSTOOL_HIP_LT_CAPI_H
 #define LENSTOOL_HIP_LT_CAPI_H

@@ -155,13 +155,14 @@ int launch_map(const DevModel& m, int kind, int mode, double* const outs[5], dou
     switch (kind) {
     case LT_MAP_GRADIENT:
     case LT_MAP_DPL:
-        e = launch_gradient64(m.recs, m.runs, m.nruns, g, o, px, s);
+        if (precision == LT_PREC_FP32C) e = launch_gradient32(m.recs, m.recs32, m.runs, m.nruns, g, o, px, s);
+        else e = launch_gradient64(m.recs, m.runs, m.nruns, g, o, px, s);
         break;
     case LT_MAP_POT:
         e = launch_potential64(m.recs, m.runs, m.nruns, g, o, s);
         break;
     default:
-        // FP32C covers the Hessian-derived maps; gradient and potential maps are FP64 either way
+        // FP32C covers the Hessian-derived and the gradient maps; potential maps are FP64 either way
         if (precision == LT_PREC_FP32C) e = launch_hessian32(m.recs, m.recs32, m.runs, m.nruns, g, o, px, s);
         else e = launch_hessian64(m.recs, m.runs, m.nruns, g, o, px, s);
         break;

@@ -74,6 +74,23 @@ struct alignas(64) HaloRec32 {
 };
 static_assert(sizeof(HaloRec32) == 128, "fp32 halo record is 128 bytes");
 
+// Single-precision record of a halo for the FP32-compensated gradient (models packed for gradient
+// use hold these in place of HaloRec32, same size). PIEMD terms use the ratio N/D of the I_{0.5}
+// gradient in the form Q/S (lt_kernels32.hip): Q = r + cx1 w + i ki x, S = w + r cx1 + kv y, with
+// w = sqrt(r^2 + kx x^2 + ky y^2); the near radius follows the HaloRec32 rule.
+struct alignas(64) GradRec32 {
+    float cs, sn;                       // 0, 1    cos, sin of the angle
+    float kx, ky, cx1;                  // 2..4    1/(1+e)^2, 1/(1-e)^2, (1-e)/(1+e)
+    float ki, kv;                       // 5, 6    2 sqrt(e)/(1+e)^2, 2 sqrt(e)/((1+e)(1-e))
+    float ra, rb, ra2, rb2;             // 7..10   rcore, rcut, their squares
+    float rca, rcb;                     // 11, 12  cx1 rcore, cx1 rcut
+    float ampg;                         // 13      gradient amplitude
+    float near2;                        // 14
+    float sa, sb, circ;                 // 15..17  SIS: 1 - e, 1 + e, 1 when circular
+    float pad[14];                      // 18..31
+};
+static_assert(sizeof(GradRec32) == sizeof(HaloRec32), "gradient fp32 record is 128 bytes");
+
 // Grid description (pixel (col, row) of the output sits at (xmin + (col+istart) dx,
 // ymin + (row+jstart) dy); output index row*pitch + col).
 struct GridDesc {

@@ -30,6 +30,9 @@ hipError_t launch_epilogue64(const double* a, const double* b, const double* c, 
 // FP32-compensated kernels (lt_kernels32.hip).
 hipError_t launch_hessian32(const HaloRec* halos64, const HaloRec32* halos32, const Run* runs, int nruns,
                             const GridDesc& g, const OutDesc& o, int px, hipStream_t s);
+// gradient / displacement maps; the model must be packed for gradient use (GradRec32 records)
+hipError_t launch_gradient32(const HaloRec* halos64, const HaloRec32* recs32, const Run* runs, int nruns,
+                             const GridDesc& g, const OutDesc& o, int px, hipStream_t s);
 
 // Streaming device copy (16-byte vectors plus a byte tail).
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);

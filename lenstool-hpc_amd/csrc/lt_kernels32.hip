@@ -333,4 +333,189 @@ hipError_t launch_hessian32(const HaloRec* h64, const HaloRec32* h32, const Run*
     }
 }
 
+// ------------------------------------------------------------------------------------------------
+// FP32-compensated gradient (gradient and displacement maps): the same scheme as the Hessian -- the
+// offset pixel - centre formed in FP64 and rounded once, FP64 terms inside each halo's near radius,
+// FP32 terms summed per chunk of CHUNK halos into FP64 run sums -- with single-precision arctangent
+// and logarithm.
+//
+// The FP64 gradient (lt_terms64.h grad_piemd) takes ln of the I_{0.5} ratio N/D with
+// N = cx1 x + i (2 s w - y / cx1), D = x + i (2 r s - y) (s = sqrt(e)); both vanish at the halo-frame
+// point (0, 2 r s), and next to it N and D are differences of nearly equal numbers, which FP32 cannot
+// carry (for the cut radius of a dPIE that point lies tens of arcseconds out). With a = 1/(1+e),
+// b = 1/(1-e), w = sqrt(r^2 + a^2 x^2 + b^2 y^2) the same ratio is N/D = Q/S with
+//   Q = r + cx1 w + 2 i s a^2 x,   S = w + r cx1 + 2 s a b y > 0
+// (w^2 - v^2 = a^2 |D|^2 for v = r cx1 + 2 s a b y, as in the Hessian's division-free form), whose
+// parts have no such cancellation. So arg(N/D) = arg Q, ln|N/D| = ln|Q| - ln S, and for the dPIE
+//   arg(Za / Zb) = atan2(I (Rb - Ra), Ra Rb + I^2),  ln|Za / Zb| = ln(|Qa|^2 Sb^2 / (|Qb|^2 Sa^2)) / 2
+// with R = Re Q and I = Im Q (the same for both radii).
+// ------------------------------------------------------------------------------------------------
+struct GFrame32 {
+    float x, y, x2, y2;
+};
+
+__device__ __forceinline__ GFrame32 gframe32(const GradRec32& h, float tx, float ty)
+{
+    GFrame32 f;
+    f.x = __builtin_fmaf(tx, h.cs, ty * h.sn);
+    f.y = __builtin_fmaf(ty, h.cs, -(tx * h.sn));
+    f.x2 = f.x * f.x;
+    f.y2 = f.y * f.y;
+    return f;
+}
+
+// R = Re Q and |Q|^2, S of one core radius
+struct QS32 {
+    float R, Q2, S;
+};
+
+__device__ __forceinline__ QS32 qs32(const GradRec32& h, float rem2, float I2, float vy, float r, float r2, float rc)
+{
+    const float q = r2 + rem2;
+    const float w = q * rsq32(q);
+    QS32 o;
+    o.R = __builtin_fmaf(h.cx1, w, r);
+    o.Q2 = __builtin_fmaf(o.R, o.R, I2);
+    o.S = w + (rc + vy);
+    return o;
+}
+
+template <bool CUT>
+__device__ __forceinline__ void grad_piemd32(const GradRec32& h, float tx, float ty, float& gx, float& gy)
+{
+    const GFrame32 f = gframe32(h, tx, ty);
+    const float rem2 = __builtin_fmaf(f.x2, h.kx, f.y2 * h.ky);
+    const float I = h.ki * f.x;
+    const float I2 = I * I;
+    const float vy = h.kv * f.y;
+    const QS32 a = qs32(h, rem2, I2, vy, h.ra, h.ra2, h.rca);
+    float arg, ratio;
+    if (CUT) {
+        const QS32 b = qs32(h, rem2, I2, vy, h.rb, h.rb2, h.rcb);
+        arg = atan2f(I * (b.R - a.R), __builtin_fmaf(a.R, b.R, I2));
+        ratio = (a.Q2 * (b.S * b.S)) * rcp32(b.Q2 * (a.S * a.S));
+    } else {
+        arg = atan2f(I, a.R);
+        ratio = a.Q2 * rcp32(a.S * a.S);
+    }
+    const float lnr = 0.5f * logf(ratio);
+    const float re = h.ampg * arg, im = -h.ampg * lnr;
+    gx += __builtin_fmaf(re, h.cs, -(im * h.sn));
+    gy += __builtin_fmaf(im, h.cs, re * h.sn);
+}
+
+__device__ __forceinline__ void grad_sis32(const GradRec32& h, float tx, float ty, float& gx, float& gy)
+{
+    const GFrame32 f = gframe32(h, tx, ty);
+    const float s = h.ampg * rsq32(__builtin_fmaf(h.sa, f.x2, h.sb * f.y2));
+    const float rx = h.sa * f.x * s, ry = h.sb * f.y * s;
+    gx += __builtin_fmaf(rx, h.cs, -(ry * h.sn));
+    gy += __builtin_fmaf(ry, h.cs, rx * h.sn);
+}
+
+__device__ __forceinline__ void grad_sis32_circ(const GradRec32& h, float tx, float ty, float& gx, float& gy)
+{
+    const float s = h.ampg * rsq32(__builtin_fmaf(tx, tx, ty * ty));
+    gx = __builtin_fmaf(s, tx, gx);
+    gy = __builtin_fmaf(s, ty, gy);
+}
+
+template <int TYPE, int PX>
+__device__ __forceinline__ void rung32(const HaloRec* __restrict__ h64, const GradRec32* __restrict__ g32, int start,
+                                       int end, const double* px, double py, double* sx, double* sy)
+{
+    for (int c0 = start; c0 < end; c0 += CHUNK) {
+        const int c1 = min(c0 + CHUNK, end);
+        float fx[PX], fy[PX];
+#pragma unroll
+        for (int k = 0; k < PX; ++k) fx[k] = fy[k] = 0.f;
+        for (int i = c0; i < c1; ++i) {
+            const GradRec32& h = g32[i];
+            const double xh = h64[i].xh;
+            const float ty = (float)(py - h64[i].yh);
+            const float ty2 = ty * ty;
+#pragma unroll
+            for (int k = 0; k < PX; ++k) {
+                const float tx = offset_x(xh, px[k]);
+                if (__builtin_fmaf(tx, tx, ty2) < h.near2) {
+                    const HaloRec& hd = h64[i];
+                    if constexpr (TYPE == TYPE_DPIE) grad_piemd<true>(hd, px[k], py, sx[k], sy[k]);
+                    else if constexpr (TYPE == TYPE_PIEMD) grad_piemd<false>(hd, px[k], py, sx[k], sy[k]);
+                    else if constexpr (TYPE == TYPE_SIS_CIRC) grad_sis_circ(hd, px[k], py, sx[k], sy[k]);
+                    else grad_sis(hd, px[k], py, sx[k], sy[k]);
+                } else {
+                    if constexpr (TYPE == TYPE_DPIE) grad_piemd32<true>(h, tx, ty, fx[k], fy[k]);
+                    else if constexpr (TYPE == TYPE_PIEMD) grad_piemd32<false>(h, tx, ty, fx[k], fy[k]);
+                    else if constexpr (TYPE == TYPE_SIS_CIRC) grad_sis32_circ(h, tx, ty, fx[k], fy[k]);
+                    else grad_sis32(h, tx, ty, fx[k], fy[k]);
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < PX; ++k) {
+            sx[k] += (double)fx[k];
+            sy[k] += (double)fy[k];
+        }
+    }
+}
+
+template <int PX>
+__global__ __launch_bounds__(WAVE* ROWS) void k_gradient32(const HaloRec* __restrict__ h64,
+                                                           const GradRec32* __restrict__ g32,
+                                                           const Run* __restrict__ runs, int nruns, GridDesc g,
+                                                           OutDesc o)
+{
+    const int row = blockIdx.y * ROWS + threadIdx.y;
+    if (row >= g.ny) return;
+    h64 = select_sample(g, h64, o);
+    g32 += (int64_t)blockIdx.z * g.halo_stride;
+    const int col0 = blockIdx.x * (WAVE * PX) + threadIdx.x;
+    const double py = pixel_coord(g.ymin, row + g.jstart, g.dy);
+    double px[PX], tx[PX], ty[PX];
+#pragma unroll
+    for (int k = 0; k < PX; ++k) {
+        px[k] = pixel_coord(g.xmin, col0 + k * WAVE + g.istart, g.dx);
+        tx[k] = ty[k] = 0.;
+    }
+    for (int r = 0; r < nruns; ++r) {
+        const Run run = runs[r];
+        double sx[PX], sy[PX];
+#pragma unroll
+        for (int k = 0; k < PX; ++k) sx[k] = sy[k] = 0.;
+        const int end = run.start + run.count;
+        if (run.type == TYPE_DPIE) rung32<TYPE_DPIE, PX>(h64, g32, run.start, end, px, py, sx, sy);
+        else if (run.type == TYPE_PIEMD) rung32<TYPE_PIEMD, PX>(h64, g32, run.start, end, px, py, sx, sy);
+        else if (run.circ) rung32<TYPE_SIS_CIRC, PX>(h64, g32, run.start, end, px, py, sx, sy);
+        else rung32<TYPE_SIS, PX>(h64, g32, run.start, end, px, py, sx, sy);
+#pragma unroll
+        for (int k = 0; k < PX; ++k) {
+            tx[k] += sx[k];
+            ty[k] += sy[k];
+        }
+    }
+    const double m = o.kind == LT_MAP_DPL ? o.r : 1.0;
+#pragma unroll
+    for (int k = 0; k < PX; ++k) {
+        const int col = col0 + k * WAVE;
+        if (col < g.nx) {
+            const int64_t idx = (int64_t)row * g.pitch + col;
+            o.p0[idx] = tx[k] * m;
+            o.p1[idx] = ty[k] * m;
+        }
+    }
+}
+
+hipError_t launch_gradient32(const HaloRec* h64, const HaloRec32* recs32, const Run* runs, int nruns,
+                             const GridDesc& g, const OutDesc& o, int px, hipStream_t s)
+{
+    const GradRec32* g32 = reinterpret_cast<const GradRec32*>(recs32);
+    dim3 blk(WAVE, ROWS, 1);
+    switch (px) {
+    case 1: hipLaunchKernelGGL(k_gradient32<1>, grid_for(g, 1), blk, 0, s, h64, g32, runs, nruns, g, o); break;
+    case 4: hipLaunchKernelGGL(k_gradient32<4>, grid_for(g, 4), blk, 0, s, h64, g32, runs, nruns, g, o); break;
+    default: hipLaunchKernelGGL(k_gradient32<2>, grid_for(g, 2), blk, 0, s, h64, g32, runs, nruns, g, o); break;
+    }
+    return hipGetLastError();
+}
+
 }  // namespace lt

@@ -2,6 +2,7 @@
 // device halo records and runs. All per-halo constants of the closed forms are formed here, once
 // per model, in double precision.
 #include <cmath>
+#include <cstring>
 #include <cstdio>
 
 #include "lenstool_hip/lt_status.h"
@@ -187,6 +188,35 @@ int pack_model(const lt_potential_soa* L, int n, Use use, std::vector<HaloRec>& 
             f.near2 = (float)(R * R);
         } else {
             f.near2 = 3.0e38f;  // type 14 stays FP64 (a constant per halo)
+        }
+        if (use == Use::Gradient) {
+            // gradient models carry the gradient's FP32 constants instead (same near radius)
+            const float near2 = f.near2;
+            GradRec32 g{};
+            g.cs = (float)cs;
+            g.sn = (float)sn;
+            g.near2 = near2;
+            if (piemd) {
+                const double a = 1.0 / (1.0 + eps), b = 1.0 / (1.0 - eps), sqe = std::sqrt(eps);
+                g.kx = (float)h.kx;
+                g.ky = (float)h.ky;
+                g.cx1 = (float)h.cx1;
+                g.ki = (float)(2.0 * sqe * a * a);
+                g.kv = (float)(2.0 * sqe * a * b);
+                g.ra = (float)h.rc;
+                g.rb = (float)h.rcut;
+                g.ra2 = (float)h.ra2;
+                g.rb2 = (float)h.rb2;
+                g.rca = (float)(h.cx1 * h.rc);
+                g.rcb = (float)(h.cx1 * h.rcut);
+                g.ampg = (float)h.ampg;
+            } else if (t == TYPE_SIS) {
+                g.sa = (float)h.sa;
+                g.sb = (float)h.sb;
+                g.ampg = (float)h.ampg;
+                g.circ = (float)h.circ;
+            }
+            std::memcpy(&f, &g, sizeof(g));
         }
     }
     // runs: equal type (and, for the Kmap, equal dlsds); the scan never reads past n - 1

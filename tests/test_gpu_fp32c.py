@@ -108,11 +108,12 @@ def test_fp32c_kmap_and_map_kinds(hip, oracle):
                 assert np.abs(1 / m32 - 1 / m64).max() <= DET_TOL
             else:
                 assert np.abs(m32 - m64).max() <= 1e-6 * max(1.0, np.abs(m64).max() / 1e3)
-    # gradient / potential requests with FP32C are served in FP64 (no type-14 gradient)
+    # displacement maps in FP32C come from the FP32C gradient kernel (bound of
+    # tests/test_gpu_fp32c_gradient.py, 2e-5 arcsec on the deflection; no type-14 gradient)
     glens = lens.subset(np.nonzero(lens.type != 14)[0])
     g32 = hip.map_grid(hip.MAP_DPL, 1, glens, grid, cosmo, 0.4, 2.0, precision=hip.PREC_FP32C)
     g64 = hip.map_grid(hip.MAP_DPL, 1, glens, grid, cosmo, 0.4, 2.0)
-    np.testing.assert_array_equal(g32[0], g64[0])
+    assert np.hypot(g32[0] - g64[0], g32[1] - g64[1]).max() <= 2e-5
 
 
 def test_fp32c_config2_full_size(hip):
