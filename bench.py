@@ -434,6 +434,7 @@ def main():
         gsteps = min(a.steps, 10)
         grads = {}
         for gname, gprec in (("fp64", lthip.PREC_FP64), ("fp32c", lthip.PREC_FP32C)):
+            gm.autotune(lthip.MAP_GRADIENT, 0, grid, gprec, stream)
             perg, totalg, _ = run_maps(gm, outs[:2], grid, 1.0, gprec, gsteps, 1, stream, 0, lthip.MAP_GRADIENT)
             grads[gname] = [o.download() for o in outs[:2]]
             res["gradient_" + gname] = {

@@ -3,7 +3,7 @@ mathematics with mpmath (Chebyshev interpolation at 50 significant digits), prin
 hexadecimal floating-point literals. Not part of the build: the printed tables are in lt_math.h;
 run `python gen_poly.py` to reproduce them.
 
-  atan(t) = t + t^3 P(t^2),            |t| <= tan(pi/8),   P of degree 10
+  atan(t) = t + t^3 P(t^2),            |t| <= tan(pi/8),   P of degree 10 (FP64), 4 (FP32: kAtanP32)
   log(m)  = 2 s + 2 s^3 R(s^2),        s = (m - 1)/(m + 1), m in [sqrt(1/2), sqrt(2)),  R of degree 6
 
 The interpolation error (printed) is far below the rounding of a double."""
@@ -41,6 +41,15 @@ def main():
         for v in c:
             print(f"    {float(v).hex()},  // {mp.nstr(v, 20)}")
         print("};")
+    # FP32 table: the coefficients rounded to float (struct round trip)
+    import struct
+    c, err = fit(P, a * a, 4)
+    print(f"// kAtanP32: degree 4 on [0, {mp.nstr(a * a, 6)}], error in atan {mp.nstr(err * a ** 3, 3)}")
+    print("constexpr float kAtanP32[5] = {")
+    for v in c:
+        f32 = struct.unpack("f", struct.pack("f", float(v)))[0]
+        print(f"    {float(f32).hex()}f,  // {mp.nstr(v, 12)}")
+    print("};")
 
 
 if __name__ == "__main__":

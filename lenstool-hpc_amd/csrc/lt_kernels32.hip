@@ -337,7 +337,7 @@ hipError_t launch_hessian32(const HaloRec* h64, const HaloRec32* h32, const Run*
 // FP32-compensated gradient (gradient and displacement maps): the same scheme as the Hessian -- the
 // offset pixel - centre formed in FP64 and rounded once, FP64 terms inside each halo's near radius,
 // FP32 terms summed per chunk of CHUNK halos into FP64 run sums -- with single-precision arctangent
-// and logarithm.
+// and logarithm (lt_math.h atan2f_lt, logf_lt).
 //
 // The FP64 gradient (lt_terms64.h grad_piemd) takes ln of the I_{0.5} ratio N/D with
 // N = cx1 x + i (2 s w - y / cx1), D = x + i (2 r s - y) (s = sqrt(e)); both vanish at the halo-frame
@@ -392,13 +392,13 @@ __device__ __forceinline__ void grad_piemd32(const GradRec32& h, float tx, float
     float arg, ratio;
     if (CUT) {
         const QS32 b = qs32(h, rem2, I2, vy, h.rb, h.rb2, h.rcb);
-        arg = atan2f(I * (b.R - a.R), __builtin_fmaf(a.R, b.R, I2));
+        arg = atan2f_lt(I * (b.R - a.R), __builtin_fmaf(a.R, b.R, I2));
         ratio = (a.Q2 * (b.S * b.S)) * rcp32(b.Q2 * (a.S * a.S));
     } else {
-        arg = atan2f(I, a.R);
+        arg = atan2f_lt(I, a.R);
         ratio = a.Q2 * rcp32(a.S * a.S);
     }
-    const float lnr = 0.5f * logf(ratio);
+    const float lnr = 0.5f * logf_lt(ratio);
     const float re = h.ampg * arg, im = -h.ampg * lnr;
     gx += __builtin_fmaf(re, h.cs, -(im * h.sn));
     gy += __builtin_fmaf(im, h.cs, re * h.sn);

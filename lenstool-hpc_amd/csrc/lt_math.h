@@ -116,6 +116,41 @@ __device__ __forceinline__ double atan2_lt(double y, double x)
     return __builtin_isunordered(x, y) ? x + y : r;
 }
 
+// FP32 arctangent for the FP32-compensated gradient: the folding of atan2_lt, then
+// atan(t) = t + t^3 P(t^2) with P of degree 4 (kAtanP32, gen_poly.py; interpolation error 1.1e-9
+// in atan, below FP32 rounding); finite arguments.
+constexpr float kAtanP32[5] = {
+    -0x1.555554p-2f,  // -0.333333317612
+    0x1.99973p-3f,    // 0.199995404836
+    -0x1.242036p-3f,  // -0.14263955598
+    0x1.b8103p-4f,    // 0.107437314908
+    -0x1.08455ep-4f,  // -0.0645192820812
+};
+
+__device__ __forceinline__ float atan2f_lt(float y, float x)
+{
+    constexpr float kPi = 3.14159265f, kPi2 = 1.57079633f, kPi4 = 0.785398163f, kTanPi8 = 0.414213562f;
+    const float ax = __builtin_fabsf(x), ay = __builtin_fabsf(y);
+    const float mx = __builtin_fmaxf(ax, ay), mn = __builtin_fminf(ax, ay);
+    const bool rot = mn > kTanPi8 * mx;
+    const float u = rot ? mx + mn : mx;
+    const float v = rot ? mn - mx : mn;
+    const float t = v * __builtin_amdgcn_rcpf(u);
+    const float z = t * t;
+    float p = kAtanP32[4];
+#pragma unroll
+    for (int k = 3; k >= 0; --k) p = __builtin_fmaf(p, z, kAtanP32[k]);
+    float r = __builtin_fmaf(t * z, p, t);
+    r = rot ? r + kPi4 : r;
+    r = ay > ax ? kPi2 - r : r;
+    r = __builtin_signbitf(x) ? kPi - r : r;
+    r = mx == 0.f ? (__builtin_signbitf(x) ? kPi : 0.f) : r;
+    return __builtin_copysignf(r, y);
+}
+
+// natural logarithm in FP32 of a normal positive x: the hardware base-2 logarithm times ln 2
+__device__ __forceinline__ float logf_lt(float x) { return __builtin_amdgcn_logf(x) * 0.693147181f; }
+
 // natural logarithm in FP64 for finite x > 0, within 3 units in the last place (2.4 measured):
 // x = m 2^e with m in [sqrt(1/2), sqrt 2), log m = 2 atanh(s) = 2 s + 2 s^3 R(s^2) with
 // s = (m - 1)/(m + 1) (|s| <= 0.1716, one corrected reciprocal), plus e ln 2 with ln 2 split so that
