@@ -110,7 +110,7 @@ def bench_bayes(a):
     amplification map (mode 1, mu), all samples in one kernel launch inside a captured graph that
     also uploads the staged parameters. A step = stage a new sample set on the host + replay."""
     S, n = a.samples, 2048
-    base = M.config1_lens(seed=1)
+    base = M.group_lens(seed=1)
     sets = [M.perturbed_samples(base, S, seed=100 + k) for k in range(2)]
     grid = M.Grid(M.grid_param(-100.0, 100.0, -100.0, 100.0), n, n)
     r = lthip.cosmo_dlsds(0.4, 2.0, M.flat_lcdm(70.0, 0.3))
@@ -188,7 +188,7 @@ def bench_chi(a):
     chi^2 path (its restatement of the reference's gradient grid, delensing and chi_computation) on a
     few of the same samples."""
     rng = np.random.default_rng(4)
-    lens = M.config1_lens(seed=1)
+    lens = M.group_lens(seed=1)
     cosmo = M.flat_lcdm(70.0, 0.3)
     frame = M.grid_param(-100.0, 100.0, -100.0, 100.0)
     fine = M.Grid(frame, 2048, 2048)
@@ -258,7 +258,7 @@ def bench_serve(a):
     pool = []
     for k in range(32):
         H = int(10 ** rng.uniform(1.0, np.log10(2000.0)))
-        L = M.config2_lens(seed=1000 + k, ngal=max(H - 3, 0))
+        L = M.cluster_lens(seed=1000 + k, ngal=max(H - 3, 0))
         L.dlsds[:] = 0.7
         L._soa = None
         pool.append(L)
@@ -339,7 +339,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); the lens-map path has no "
-              "multi-GPU code in this build", file=sys.stderr)
+              "multi-GPU code in this library", file=sys.stderr)
         sys.exit(2)
     lthip.check(lthip.lib().lt_init(0))
     if a.workload != "cluster":
@@ -349,8 +349,8 @@ def main():
         print(json.dumps(fn(a)), flush=True)
         return
 
-    lens = M.config2_lens(seed=2)
-    grid = M.config2_grid(4096)
+    lens = M.cluster_lens(seed=2)
+    grid = M.cluster_grid(4096)
     cosmo = M.flat_lcdm(70.0, 0.3)
     zl, zs = 0.4, 2.0
     r = lthip.cosmo_dlsds(zl, zs, cosmo)

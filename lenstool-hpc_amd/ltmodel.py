@@ -228,7 +228,7 @@ class Grid:
         return self.nx * self.ny
 
 
-def config1_lens(seed: int = 1) -> LensModel:
+def group_lens(seed: int = 1) -> LensModel:
     """One cluster dPIE (sigma 1000 km/s, e ~ U[0,0.5], theta ~ U[0,pi), rcore 10", rcut 1000") at
     the origin plus ten SIS galaxies (sigma lognormal, median 150 km/s, 0.3 dex; positions uniform
     in the 200" field)."""
@@ -243,11 +243,11 @@ def config1_lens(seed: int = 1) -> LensModel:
     return LensModel.from_halos(halos)
 
 
-def config1_grid(n: int = 512) -> Grid:
+def group_grid(n: int = 512) -> Grid:
     return Grid(grid_param(-100.0, 100.0, -100.0, 100.0), n, n)
 
 
-def config2_lens(seed: int = 2, ngal: int = 500) -> LensModel:
+def cluster_lens(seed: int = 2, ngal: int = 500) -> LensModel:
     """Two cluster dPIEs + one BCG dPIE (sigma 1000/800/300 km/s, rcore 10/8/0.5", rcut
     1000/1000/100", e ~ U[0,0.6]) plus `ngal` galaxy dPIEs scaled from magnitudes m ~ U[18,24]
     with mag0 = 18, slopes 4: sigma = 180 x 10^(0.4(18-m)/4), rcut = 40" x 10^(0.4(18-m)/2),
@@ -270,7 +270,7 @@ def config2_lens(seed: int = 2, ngal: int = 500) -> LensModel:
     return LensModel.from_halos(halos)
 
 
-def config2_grid(n: int = 4096) -> Grid:
+def cluster_grid(n: int = 4096) -> Grid:
     return Grid(grid_param(-150.0, 150.0, -150.0, 150.0), n, n)
 
 

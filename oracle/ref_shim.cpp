@@ -3,7 +3,7 @@
 // chi_computation.cpp). TEST INFRASTRUCTURE ONLY: it is used to write golden vectors and to
 // validate the oracle restatement; nothing in the product library links it.
 //
-// gradient.cpp (the reference gradient) is not part of this build: it does not compile unmodified
+// gradient.cpp (the reference gradient) is not compiled into ref_shim: it does not compile unmodified
 // (a `switch` jumps past an initialised variable in the unused AOS grad_halo, gradient.cpp:152-157).
 #include <cstring>
 #include <string>

@@ -81,7 +81,7 @@ def main():
     np.savez(os.path.join(HERE, "hess81_points.npz"), **lens_arrays(L), points=pts, hessian=H)
 
     # 3. delensing -------------------------------------------------------------------------------
-    lens1 = M.config1_lens(seed=5)
+    lens1 = M.group_lens(seed=5)
     n = 64
     dframe = M.grid_param(-100.0, 100.0, -100.0, 100.0)
     grid = M.Grid(dframe, n, n)

@@ -56,8 +56,8 @@ def test_invalid_arguments_without_gpu(lthip_lib):
     """Argument validation runs before any device call."""
     import ctypes as C
     import lthip
-    lens = M.config1_lens()
-    grid = M.config1_grid(8)
+    lens = M.group_lens()
+    grid = M.group_grid(8)
     bad = lens.subset(range(lens.n))
     bad.type[0] = 42
     bad._soa = None

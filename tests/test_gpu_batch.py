@@ -31,9 +31,9 @@ def _single(hip, lens, kind, mode, grid, r, precision, stream):
 
 @pytest.mark.parametrize("precision", [0, 1])
 def test_batch_equals_single(hip, precision):
-    base = M.config1_lens(seed=4)
+    base = M.group_lens(seed=4)
     samples = M.perturbed_samples(base, 6, seed=11)
-    grid = M.config1_grid(192)
+    grid = M.group_grid(192)
     print_case(f"batch of 6 samples precision={precision}", base, grid)
     s = hip.stream_create()
     try:
@@ -59,9 +59,9 @@ def test_batch_equals_single(hip, precision):
 def test_graph_replays_upload_and_map(hip):
     """Capture [upload staged parameters, fields map]; replay it for three different staged samples
     with eager work on the same stream between replays; every replay equals the eager map."""
-    base = M.config1_lens(seed=4)
+    base = M.group_lens(seed=4)
     samples = M.perturbed_samples(base, 3, seed=12)
-    grid = M.config1_grid(160)
+    grid = M.group_grid(160)
     print_case("graph replay upload + fields", base, grid)
     s = hip.stream_create()
     try:
@@ -94,8 +94,8 @@ def test_graph_replays_upload_and_map(hip):
 
 
 def test_batch_rejects_mismatched_samples(hip):
-    base = M.config1_lens(seed=4)
-    other = M.config1_lens(seed=5)
+    base = M.group_lens(seed=4)
+    other = M.group_lens(seed=5)
     other.type[3] = 81
     other.rcore[3] = 0.2
     other.rcut[3] = 5.0
@@ -108,8 +108,8 @@ def test_batch_rejects_mismatched_samples(hip):
 @pytest.mark.parametrize("precision", [0, 1])
 def test_autotune_keeps_results(hip, precision):
     """The autotuner only changes the launch shape: maps before and after tuning are identical."""
-    lens = M.config2_lens(seed=8, ngal=30)
-    grid = M.config2_grid(384)
+    lens = M.cluster_lens(seed=8, ngal=30)
+    grid = M.cluster_grid(384)
     print_case(f"autotune precision={precision}", lens, grid)
     s = hip.stream_create()
     try:

@@ -137,7 +137,7 @@ def test_chi_computation_golden(lt, oracle):
 
 
 def test_point_gradients(lt, oracle):
-    lens = M.config2_lens(seed=4, ngal=60)
+    lens = M.cluster_lens(seed=4, ngal=60)
     rng = np.random.default_rng(8)
     px = rng.uniform(-150, 150, 777)
     py = rng.uniform(-150, 150, 777)
@@ -185,11 +185,11 @@ def _families(lens, oracle, n, frame, nsets, seed, per=3):
 
 
 def test_chi_barycentersource(lt, oracle):
-    lens = M.config1_lens(seed=5)
+    lens = M.group_lens(seed=5)
     n = 96
     frame = M.grid_param(-100.0, 100.0, -100.0, 100.0)
     nimages, images = _families(lens, oracle, n, frame, 12, seed=21)
-    _case(f"chi_barycentersource config1 n={n} 12 families")
+    _case(f"chi_barycentersource group lens n={n} 12 families")
     st, chi = lt.chi_barycentersource(lens, n, frame, nimages, images)
     ost, ochi = oracle.chi_barycentersource(lens, n, frame, nimages, images)
     assert st == ost == 0
@@ -200,7 +200,7 @@ def test_chi_barycentersource(lt, oracle):
 def test_chi_batch_matches_single(lt, oracle):
     """A batch of perturbed lens samples in one call: each sample's chi^2 and delensed images equal
     the single-model path (bitwise) and the oracle (within the gradient tolerance)."""
-    base = M.config1_lens(seed=5)
+    base = M.group_lens(seed=5)
     n = 80
     frame = M.grid_param(-100.0, 100.0, -100.0, 100.0)
     nimages, images = _families(base, oracle, n, frame, 9, seed=22)
@@ -232,7 +232,7 @@ def test_chi_batch_matches_single(lt, oracle):
 
 
 def test_chi_rejects_bad_families(lt):
-    lens = M.config1_lens(seed=5)
+    lens = M.group_lens(seed=5)
     frame = M.grid_param(-100.0, 100.0, -100.0, 100.0)
     images = M.galaxies_array(3)
     with pytest.raises(lt.LtError):

@@ -116,19 +116,19 @@ def test_fp32c_kmap_and_map_kinds(hip, oracle):
     assert np.hypot(g32[0] - g64[0], g32[1] - g64[1]).max() <= 2e-5
 
 
-def test_fp32c_config2_full_size(hip):
+def test_fp32c_cluster_full_size(hip):
     """The cluster config at full size (4096^2, 503 dPIE halos): det A of the FP32C fields map within
     1e-6 of the FP64 map at every pixel; kappa and gamma within 2e-7."""
-    lens = M.config2_lens()
-    grid = M.config2_grid(4096)
-    print_case("fp32c config2 full size", lens, grid)
+    lens = M.cluster_lens()
+    grid = M.cluster_grid(4096)
+    print_case("fp32c cluster lens full size", lens, grid)
     cosmo = M.flat_lcdm()
     f64 = hip.map_fields(lens, grid, cosmo, 0.4, 2.0)
     f32 = hip.map_fields(lens, grid, cosmo, 0.4, 2.0, precision=hip.PREC_FP32C)
     err = {}
     for name, p32, p64 in zip(("kappa", "gamma1", "gamma2", "detA"), f32[:4], f64[:4]):
         err[name] = float(np.abs(p32 - p64).max())
-    print("[fp32c config2] max abs errors:", err, flush=True)
+    print("[fp32c cluster lens] max abs errors:", err, flush=True)
     assert err["detA"] <= DET_TOL
     for k in ("kappa", "gamma1", "gamma2"):
         assert err[k] <= HESS_TOL

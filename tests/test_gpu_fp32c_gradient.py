@@ -34,8 +34,8 @@ def _max_err(hip, lens, grid):
 
 
 def test_cluster_lens(hip):
-    lens = M.config2_lens(seed=2)
-    grid = M.config2_grid(1024)
+    lens = M.cluster_lens(seed=2)
+    grid = M.cluster_grid(1024)
     print_case("FP32C gradient, cluster lens", lens, grid)
     err = _max_err(hip, lens, grid)
     print(f"max |dalpha| = {err:.3e} arcsec", flush=True)
@@ -73,9 +73,9 @@ def test_cut_radius_removable_point(hip):
 
 
 def test_batch_and_displacement(hip):
-    base = M.config1_lens(seed=4)
+    base = M.group_lens(seed=4)
     samples = M.perturbed_samples(base, 3, seed=13)
-    grid = M.config1_grid(160)
+    grid = M.group_grid(160)
     print_case("FP32C gradient batch of 3 and displacement", base, grid)
     s = hip.stream_create()
     try:

@@ -24,7 +24,7 @@ def hip():
 
 
 def test_bayes_batch_fullsize(hip):
-    base = M.config1_lens(seed=1)
+    base = M.group_lens(seed=1)
     samples = M.perturbed_samples(base, S, seed=100)
     grid = M.Grid(M.grid_param(-100.0, 100.0, -100.0, 100.0), N, N)
     r = hip.cosmo_dlsds(0.4, 2.0, M.flat_lcdm(70.0, 0.3))

@@ -52,7 +52,7 @@ def _catalogue(lt, oracle, lens, frame, cosmo):
 
 
 def test_chi_fullsize_batch(lt, oracle):
-    base = M.config1_lens(seed=1)
+    base = M.group_lens(seed=1)
     cosmo = M.flat_lcdm(70.0, 0.3)
     frame = M.grid_param(-100.0, 100.0, -100.0, 100.0)
     nimages, images = _catalogue(lt, oracle, base, frame, cosmo)

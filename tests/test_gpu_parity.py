@@ -70,10 +70,10 @@ def test_gradient_mixed(hip, oracle, istart, jstart):
     assert_close("gy", gy, ey, ay, cg)
 
 
-def test_gradient_config1(hip, oracle):
-    lens = M.config1_lens(seed=1)
-    grid = M.config1_grid(128)
-    print_case("gradient config1", lens, grid)
+def test_gradient_group_lens(hip, oracle):
+    lens = M.group_lens(seed=1)
+    grid = M.group_grid(128)
+    print_case("gradient group lens", lens, grid)
     gx, gy = hip.gradient_grid(lens, grid)
     ex, ey = oracle.gradient_grid(lens, grid, extended=True)
     _, _, ax, ay = oracle.gradient_grid(lens, grid, with_abs=True)
@@ -197,8 +197,8 @@ def _epilogue_reference(oracle, kind, mode, lens, grid, cosmo, zl, zs, kmap=Fals
 @pytest.mark.parametrize("kind,mode", [(1, m) for m in range(1, 7)] + [(2, m) for m in range(1, 5)] +
                          [(3, m) for m in range(1, 5)])
 def test_maps_reference_modes(hip, oracle, kind, mode):
-    lens = M.config1_lens(seed=4)
-    grid = M.config1_grid(96)
+    lens = M.group_lens(seed=4)
+    grid = M.group_grid(96)
     cosmo = M.flat_lcdm()
     zl, zs = 0.4, 2.0
     if kind == 3:
@@ -228,8 +228,8 @@ def test_maps_reference_modes(hip, oracle, kind, mode):
 
 
 def test_map_fields_and_dpl(hip, oracle):
-    lens = M.config1_lens(seed=4)
-    grid = M.config1_grid(96)
+    lens = M.group_lens(seed=4)
+    grid = M.group_grid(96)
     cosmo = M.flat_lcdm()
     zl, zs = 0.4, 2.0
     kap, g1, g2, det, mu = hip.map_fields(lens, grid, cosmo, zl, zs)
@@ -268,13 +268,13 @@ def test_potential(hip, oracle):
 # full-size (4096^2, 503 halos) properties
 # ------------------------------------------------------------------------------------------------
 
-def test_config2_fullsize_sampled(hip, oracle):
+def test_cluster_fullsize_sampled(hip, oracle):
     """The cluster Hessian config at full size: the GPU grid at 2000 random pixels equals the oracle's
     point evaluations there, and rows are independent of the grid they are computed in (a sub-grid
     with istart/jstart offsets reproduces the same pixels bit for bit)."""
-    lens = M.config2_lens()
-    grid = M.config2_grid(4096)
-    print_case("config2 full size", lens, grid)
+    lens = M.cluster_lens()
+    grid = M.cluster_grid(4096)
+    print_case("cluster lens full size", lens, grid)
     a, b, c, d = hip.hessian_grid(lens, grid)
     assert np.isfinite(a).all() and np.isfinite(c).all()
     rng = np.random.default_rng(1)

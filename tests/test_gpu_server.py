@@ -17,9 +17,9 @@ def hip():
 
 
 def _pool():
-    a = M.config2_lens(seed=5, ngal=40)
-    b = M.config1_lens(seed=6)
-    c = M.config2_lens(seed=7, ngal=120)
+    a = M.cluster_lens(seed=5, ngal=40)
+    b = M.group_lens(seed=6)
+    c = M.cluster_lens(seed=7, ngal=120)
     for L in (a, b, c):
         L.dlsds[:] = 0.7
         L._soa = None

@@ -20,13 +20,13 @@ def test_reference_signatures(oracle):
     lthip.check(lthip.lib().lt_init(0))
     if not os.path.exists(BIN):
         pytest.fail("tests/cpp/shim_check is not built (make -C lenstool-hpc_amd/csrc)")
-    lens = M.config1_lens(seed=9)
+    lens = M.group_lens(seed=9)
     cosmo = M.flat_lcdm()
     zs = 2.0
     lens.dlsds[:] = oracle.cosmo_dlsds(0.4, 1.7, cosmo)
     lens._soa = None
     n = 72
-    grid = M.config1_grid(n)
+    grid = M.group_grid(n)
     with tempfile.TemporaryDirectory() as d:
         fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
         with open(fin, "wb") as f:
@@ -34,7 +34,7 @@ def test_reference_signatures(oracle):
             f.write(lens.type.astype("<i4").tobytes())
             for a in (lens.x, lens.y, lens.b0, lens.angle, lens.epot, lens.rcore, lens.rcut, lens.z, lens.dlsds):
                 f.write(a.astype("<f8").tobytes())
-        print(f"[case] shim_check config1 {lens.n} halos {n}x{n}", flush=True)
+        print(f"[case] shim_check group lens {lens.n} halos {n}x{n}", flush=True)
         subprocess.run([BIN, fin, fout], check=True, timeout=120)
         raw = open(fout, "rb").read()
     status = struct.unpack("<i", raw[-4:])[0]
