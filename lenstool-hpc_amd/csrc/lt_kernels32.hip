@@ -36,6 +36,45 @@ namespace lt {
 
 constexpr int CHUNK = 8;
 
+// The gradient kernel stages the FP32 records of a run through LDS in tiles of TILE records (8 KB),
+// which the four waves of the work-group then read with broadcast LDS loads. On gfx950 an FP32 VALU
+// instruction with an SGPR operand issues at half the rate of one whose operands are all VGPRs
+// (measured on independent chains: 0.57 against 1.0-1.09 wave instructions per ns per SIMD; FP64
+// issues at 0.57 either way), and records read from LDS arrive in VGPRs where scalar loads of a
+// wavefront-uniform record leave them in SGPRs. TILE is a multiple of CHUNK, so the chunks (and the
+// sums) are those of the untiled walk.
+constexpr int TILE = 64;
+static_assert(TILE % CHUNK == 0, "tiles hold whole chunks");
+
+// n records recs[i0 .. i0+n) into tile[0 .. n), 16 bytes per thread and step; the caller brackets
+// it with barriers (every thread of the work-group calls it with the same arguments)
+template <typename Rec>
+__device__ __forceinline__ void stage_tile(Rec* __restrict__ tile, const Rec* __restrict__ recs, int i0, int n)
+{
+    static_assert(sizeof(Rec) % 16 == 0, "records are whole 16-byte vectors");
+    constexpr int V = sizeof(Rec) / 16;
+    const float4* src = reinterpret_cast<const float4*>(recs + i0);
+    float4* dst = reinterpret_cast<float4*>(tile);
+    for (int j = threadIdx.y * WAVE + threadIdx.x; j < n * V; j += WAVE * ROWS) dst[j] = src[j];
+}
+
+// Record j of a tile by value, only its first NV 16-byte vectors read (the fields the FP32 terms
+// use; the rest of the copy is left unset and never read): one set of wide LDS loads per halo at
+// the top of its iteration, which both pixels' branches then share.
+template <int NV, typename Rec>
+__device__ __forceinline__ Rec hot_fields(const Rec* __restrict__ tile, int j)
+{
+    static_assert(NV * 16 <= (int)sizeof(Rec), "within the record");
+    Rec r;
+    const float4* src = reinterpret_cast<const float4*>(tile + j);
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        const float4 v = src[q];
+        __builtin_memcpy(reinterpret_cast<char*>(&r) + 16 * q, &v, 16);
+    }
+    return r;
+}
+
 __device__ __forceinline__ float rsq32(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ float rcp32(float x) { return __builtin_amdgcn_rcpf(x); }
 
@@ -163,7 +202,9 @@ __device__ __forceinline__ void hess_sis32_circ(const HaloRec32& h, float tx, fl
 constexpr int TYPE_SIS_CIRC = -5;
 
 // One run of halos of one type: FP32 terms summed per chunk of CHUNK halos, FP64 terms (near
-// lanes) straight into the FP64 run sums.
+// lanes) straight into the FP64 run sums. The records are wavefront-uniform scalar loads (the
+// Hessian keeps them in SGPRs: staged through LDS they cost 20 VGPRs, two waves of occupancy, and
+// measured slower, 9.6 against 8.9 ms on the 503-halo 4096^2 map).
 template <int TYPE, int PX>
 __device__ __forceinline__ void run32(const HaloRec* __restrict__ h64, const HaloRec32* __restrict__ h32, int start,
                                       int end, const double* px, double py, double* sT, double* sD, double* sB)
@@ -420,17 +461,19 @@ __device__ __forceinline__ void grad_sis32_circ(const GradRec32& h, float tx, fl
     gy = __builtin_fmaf(s, ty, gy);
 }
 
+// Halos [t0, t1) of a run, their FP32 records in tile[0 .. t1 - t0): FP32 terms summed per chunk of
+// CHUNK halos, FP64 terms (near lanes) straight into the FP64 run sums
 template <int TYPE, int PX>
-__device__ __forceinline__ void rung32(const HaloRec* __restrict__ h64, const GradRec32* __restrict__ g32, int start,
-                                       int end, const double* px, double py, double* sx, double* sy)
+__device__ __forceinline__ void tileg32(const HaloRec* __restrict__ h64, const GradRec32* __restrict__ tile, int t0,
+                                        int t1, const double* px, double py, double* sx, double* sy)
 {
-    for (int c0 = start; c0 < end; c0 += CHUNK) {
-        const int c1 = min(c0 + CHUNK, end);
+    for (int c0 = t0; c0 < t1; c0 += CHUNK) {
+        const int c1 = min(c0 + CHUNK, t1);
         float fx[PX], fy[PX];
 #pragma unroll
         for (int k = 0; k < PX; ++k) fx[k] = fy[k] = 0.f;
         for (int i = c0; i < c1; ++i) {
-            const GradRec32& h = g32[i];
+            const GradRec32 h = hot_fields<5>(tile, i - t0);  // cs .. circ
             const double xh = h64[i].xh;
             const float ty = (float)(py - h64[i].yh);
             const float ty2 = ty * ty;
@@ -459,14 +502,33 @@ __device__ __forceinline__ void rung32(const HaloRec* __restrict__ h64, const Gr
     }
 }
 
+// One run of halos of one type, its FP32 records staged tile by tile (staged: 12.9 ms on the
+// 503-halo 4096^2 map, scalar loads: 13.9 ms). Called by every thread of the work-group with the
+// same run (the barriers need all of them).
+template <int TYPE, int PX>
+__device__ __forceinline__ void rung32(const HaloRec* __restrict__ h64, const GradRec32* __restrict__ g32,
+                                       GradRec32* __restrict__ tile, int start, int end, const double* px, double py,
+                                       double* sx, double* sy)
+{
+    for (int t0 = start; t0 < end; t0 += TILE) {
+        const int t1 = min(t0 + TILE, end);
+        __syncthreads();  // every wave is done with the previous tile
+        stage_tile(tile, g32, t0, t1 - t0);
+        __syncthreads();
+        tileg32<TYPE, PX>(h64, tile, t0, t1, px, py, sx, sy);
+    }
+}
+
 template <int PX>
 __global__ __launch_bounds__(WAVE* ROWS) void k_gradient32(const HaloRec* __restrict__ h64,
                                                            const GradRec32* __restrict__ g32,
                                                            const Run* __restrict__ runs, int nruns, GridDesc g,
                                                            OutDesc o)
 {
+    // rows past the grid walk the runs too (the tile barriers need every thread) and store nothing
+    __shared__ GradRec32 tile[TILE];
     const int row = blockIdx.y * ROWS + threadIdx.y;
-    if (row >= g.ny) return;
+    const bool live = row < g.ny;
     h64 = select_sample(g, h64, o);
     g32 += (int64_t)blockIdx.z * g.halo_stride;
     const int col0 = blockIdx.x * (WAVE * PX) + threadIdx.x;
@@ -483,10 +545,10 @@ __global__ __launch_bounds__(WAVE* ROWS) void k_gradient32(const HaloRec* __rest
 #pragma unroll
         for (int k = 0; k < PX; ++k) sx[k] = sy[k] = 0.;
         const int end = run.start + run.count;
-        if (run.type == TYPE_DPIE) rung32<TYPE_DPIE, PX>(h64, g32, run.start, end, px, py, sx, sy);
-        else if (run.type == TYPE_PIEMD) rung32<TYPE_PIEMD, PX>(h64, g32, run.start, end, px, py, sx, sy);
-        else if (run.circ) rung32<TYPE_SIS_CIRC, PX>(h64, g32, run.start, end, px, py, sx, sy);
-        else rung32<TYPE_SIS, PX>(h64, g32, run.start, end, px, py, sx, sy);
+        if (run.type == TYPE_DPIE) rung32<TYPE_DPIE, PX>(h64, g32, tile, run.start, end, px, py, sx, sy);
+        else if (run.type == TYPE_PIEMD) rung32<TYPE_PIEMD, PX>(h64, g32, tile, run.start, end, px, py, sx, sy);
+        else if (run.circ) rung32<TYPE_SIS_CIRC, PX>(h64, g32, tile, run.start, end, px, py, sx, sy);
+        else rung32<TYPE_SIS, PX>(h64, g32, tile, run.start, end, px, py, sx, sy);
 #pragma unroll
         for (int k = 0; k < PX; ++k) {
             tx[k] += sx[k];
@@ -497,7 +559,7 @@ __global__ __launch_bounds__(WAVE* ROWS) void k_gradient32(const HaloRec* __rest
 #pragma unroll
     for (int k = 0; k < PX; ++k) {
         const int col = col0 + k * WAVE;
-        if (col < g.nx) {
+        if (live && col < g.nx) {
             const int64_t idx = (int64_t)row * g.pitch + col;
             o.p0[idx] = tx[k] * m;
             o.p1[idx] = ty[k] * m;
