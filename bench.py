@@ -429,7 +429,7 @@ def main():
                      "fp32c_over_fp64": (v2 / value) if name == "fp32c" else (value / v2),
                      "fp32c_max_abs_err_vs_fp64": errs,
                      "fp32c_max_rel_err_vs_fp64_where_abs_detA_ge_1e-3": rel}
-        # gradient maps of the same lens and grid (FP64; FP32C gradient requests use the FP64 kernel)
+        # gradient maps of the same lens and grid in both precisions, each autotuned first
         gm = lthip.Model(lens, lthip.MAP_GRADIENT)
         gsteps = min(a.steps, 10)
         grads = {}
