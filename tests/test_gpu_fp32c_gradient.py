@@ -99,3 +99,39 @@ def test_batch_and_displacement(hip):
         bm.close()
     finally:
         hip.stream_destroy(s)
+
+
+
+def _tile_lens(n_dpie, n_sis, sis_circular, n_piemd, seed):
+    rng = np.random.default_rng(seed)
+    halos = []
+    for _ in range(n_dpie):
+        halos.append(dict(type=81, x=float(rng.uniform(-60, 60)), y=float(rng.uniform(-60, 60)),
+                          sigma=float(rng.uniform(80, 250)), e=float(rng.uniform(0, 0.5)),
+                          theta=float(rng.uniform(0, math.pi)), rcore=float(rng.uniform(0.1, 2.0)),
+                          rcut=float(rng.uniform(10, 80))))
+    for _ in range(n_sis):
+        halos.append(dict(type=5, x=float(rng.uniform(-60, 60)), y=float(rng.uniform(-60, 60)),
+                          sigma=float(rng.uniform(60, 200)),
+                          e=0.0 if sis_circular else float(rng.uniform(0.05, 0.4)),
+                          theta=float(rng.uniform(0, math.pi))))
+    for _ in range(n_piemd):
+        halos.append(dict(type=8, x=float(rng.uniform(-60, 60)), y=float(rng.uniform(-60, 60)),
+                          sigma=float(rng.uniform(80, 250)), e=float(rng.uniform(0, 0.5)),
+                          theta=float(rng.uniform(0, math.pi)), rcore=float(rng.uniform(0.1, 2.0))))
+    return M.LensModel.from_halos(halos)
+
+
+@pytest.mark.parametrize("n_dpie, n_sis, circular, n_piemd", [(64, 65, False, 7), (63, 129, True, 0)])
+def test_tile_boundaries(hip, n_dpie, n_sis, circular, n_piemd):
+    """The FP32 records of a run are staged through LDS 64 at a time. Runs (one per type) of 64 dPIE
+    (one full tile), 65 elliptical SIS (a full tile and one record) and 7 PIEMD (a partial tile), then
+    63 dPIE and a circular-SIS run of 129 (two full tiles and one record), on a 203 x 131 grid (131 rows:
+    the last work-group has rows past the grid, which stage records and store nothing)."""
+    lens = _tile_lens(n_dpie, n_sis, circular, n_piemd, seed=17 + n_sis)
+    grid = M.Grid(M.grid_param(-70, 70, -45, 45), 203, 131)
+    print_case(f"FP32C gradient across LDS tile boundaries: {n_dpie} dPIE, {n_sis} SIS"
+               f" ({'circular' if circular else 'elliptical'}), {n_piemd} PIEMD", lens, grid)
+    err = _max_err(hip, lens, grid)
+    print(f"max |dalpha| = {err:.3e} arcsec", flush=True)
+    assert err <= BOUND
