@@ -263,7 +263,10 @@ int lt_chi_batch_run(lt_chi_batch b, const lt_potential_soa* lenses, double* chi
     o.kind = LT_MAP_GRADIENT;
     o.r = 1.0;
     o.mult = 1.0;
-    LT_HIP_TRY(launch_gradient64(b->recs, b->druns, (int)b->runs.size(), g, o, 2, s));
+    // 4 pixels per thread: with few halos per sample each halo record's scalar load is shared by more
+    // pixels (1024 samples of 11 halos on 512^2: 9.9 ms per batch step against 10.4 at 2; the planes
+    // are bit-identical for every pixels-per-thread)
+    LT_HIP_TRY(launch_gradient64(b->recs, b->druns, (int)b->runs.size(), g, o, 4, s));
     LT_HIP_TRY(launch_points_gradient64(b->recs, b->druns, (int)b->runs.size(), b->d.img_x, b->d.img_y, b->nimg,
                                         b->S, b->H, b->img_gx, b->img_gy, s));
     LT_HIP_TRY(launch_barycentres(b->img_gx, b->img_gy, b->d, s));
