@@ -38,28 +38,47 @@ PEAK_FP32_TFLOPS = 157.3  # MI355X vector FP32 (published figure, counts packed 
 PEAK_HBM_GBS = 8000.0
 METRIC = "halo·pixel evals/s, Hessian maps (FP32-comp vs FP64 GPU); p50/p99 map latency"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hessian_fields_fp64_4096_503halos_traffic.md")
+COUNTERS_FILE = os.path.join(ROOT, "profiles", "hessian_fields_4096_503halos_valu_counters.md")
+SIMDS = 1024          # 256 CUs x 4 SIMDs
+CLOCK_HZ = 2.4e9      # MI355X peak engine clock
 
 
 def percentile(v, q):
     return float(np.percentile(np.asarray(v), q))
 
 
-def run_maps(model, outs, grid, r, precision, steps, warmup, stream, px=0, kind=lthip.MAP_FIELDS):
+def run_maps(model, outs, grid, r, precision, steps, warmup, stream, px=0, kind=lthip.MAP_FIELDS, mode=0):
     ptrs = [o.ptr for o in outs]
     for _ in range(warmup):
-        model.map(kind, 0, ptrs, grid, r=r, precision=precision, stream=stream, px=px)
+        model.map(kind, mode, ptrs, grid, r=r, precision=precision, stream=stream, px=px)
     lthip.sync(stream)
     ev = [lthip.Event() for _ in range(steps + 1)]
     t0 = time.perf_counter()
     ev[0].record(stream)
     for k in range(steps):
-        model.map(kind, 0, ptrs, grid, r=r, precision=precision, stream=stream, px=px)
+        model.map(kind, mode, ptrs, grid, r=r, precision=precision, stream=stream, px=px)
         ev[k + 1].record(stream)
     ev[-1].synchronize()
     wall = time.perf_counter() - t0
     per = [ev[k].elapsed_ms(ev[k + 1]) for k in range(steps)]
     total_ms = ev[0].elapsed_ms(ev[-1])
     return per, total_ms, wall
+
+
+def read_counters():
+    """Per dPIE halo x pixel figures of the FP64 Hessian + fields kernel from its committed rocprofv3
+    counter summary: hardware FP64 flops (64 lanes x (ADD + MUL + 2 FMA) from SQ_INSTS_VALU_*_F64) and
+    VALU issue slots (SQ_ACTIVE_INST_VALU); None when absent."""
+    if not os.path.exists(COUNTERS_FILE):
+        return None
+    txt = open(COUNTERS_FILE).read()
+    out = {}
+    for key in ("fp64_hw_flop_per_halo_px", "valu_issue_slots_per_halo_px"):
+        m = re.search(key + r":\s*([0-9.eE+]+)", txt)
+        if not m:
+            return None
+        out[key] = float(m.group(1))
+    return out
 
 
 def read_traffic():
@@ -71,37 +90,68 @@ def read_traffic():
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(lens, grid, budget_s=6.0):
-    """The reference's own CPU Hessian grid (oracle/_ref, gradient2_grid_CPU compiled from the
-    reference sources) when it was built, else the oracle's restatement; on whole rows of the same
-    4096-wide grid with all halos, doubling the rows until one run takes >= budget_s."""
+def cpu_model():
+    try:
+        return next(ln.split(":", 1)[1].strip() for ln in open("/proc/cpuinfo") if ln.startswith("model name"))
+    except (OSError, StopIteration):
+        return "unknown"
+
+
+def cpu_baseline(lens, grid, run_s=2.0, runs=5):
+    """The reference's own CPU Hessian grid (gradient2_grid_CPU compiled from the reference sources in
+    oracle/_ref, in its AVX2 and, where the host has AVX-512, its AVX-512 build) on whole rows of the
+    same 4096-wide grid with all halos, all host cores through OpenMP. The row count is sized so that
+    one run takes about run_s; one warm-up run, then the median of `runs` timed runs, for each build;
+    the faster build is the baseline. Falls back to the oracle's restatement when no build exists."""
     import ctypes as C
+    import statistics
     import pyoracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
-    R = O.ref_lib()
-    kind = "reference" if R is not None else "port"
-    if R is not None:
+    builds = O.ref_builds()
+
+    def runner(name):
+        if name is None:
+            O.set_threads(threads)
+            return lambda out, sub: O.lib().orc_gradient2_grid(out, C.byref(sub.frame), C.byref(lens.soa()), lens.n,
+                                                               sub.dx, sub.dy, sub.nx, sub.ny, 0, 2048)
+        R = O.ref_lib(name)
         R.ref_set_threads(threads)
-    else:
-        O.set_threads(threads)
-    rows = 2
+        return lambda out, sub: R.ref_gradient2_grid(out, C.byref(sub.frame), C.byref(lens.soa()), lens.n, sub.dx,
+                                                     sub.dy, sub.nx, sub.ny, 0, 2048)
+
+    # rows for ~run_s with the first build
+    first = runner(builds[0] if builds else None)
+    rows = 4
     while True:
         sub = M.Grid(grid.frame, grid.nx, rows, step=(grid.dx, grid.dy))
         out = (M.LtMatrix * sub.npix)()
         t0 = time.perf_counter()
-        if R is not None:
-            R.ref_gradient2_grid(out, C.byref(sub.frame), C.byref(lens.soa()), lens.n, sub.dx, sub.dy, sub.nx,
-                                 sub.ny, 0, 2048)
-        else:
-            O.lib().orc_gradient2_grid(out, C.byref(sub.frame), C.byref(lens.soa()), lens.n, sub.dx, sub.dy,
-                                       sub.nx, sub.ny, 0, 2048)
+        first(out, sub)
         dt = time.perf_counter() - t0
-        if dt >= budget_s or rows >= grid.ny:
+        if dt >= 0.25 * run_s or rows >= 1024:
             break
-        rows = min(grid.ny, rows * max(2, int(budget_s / max(dt, 1e-3) * 0.6)))
-    return {"value": lens.n * sub.npix / dt, "unit": "halo·px/s", "cores": threads, "kind": kind,
-            "sample": f"Hessian grid of all {lens.n} halos on {rows} full rows (4096 px) of the same "
-                      f"grid, rows 2048.., {dt:.1f} s, OMP threads = {threads}"}
+        rows *= 4
+    rows = int(min(grid.ny - 2048, max(4, rows * run_s / max(dt, 1e-3))))
+    sub = M.Grid(grid.frame, grid.nx, rows, step=(grid.dx, grid.dy))
+    out = (M.LtMatrix * sub.npix)()
+    per_build = {}
+    for name in (builds or [None]):
+        f = runner(name)
+        f(out, sub)  # warm-up
+        ts = []
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            f(out, sub)
+            ts.append(time.perf_counter() - t0)
+        per_build[name or "oracle port"] = lens.n * sub.npix / statistics.median(ts)
+    best = max(per_build, key=per_build.get)
+    flags = O.REF_FLAGS.get(best, "gcc -O3 -march=x86-64-v3 -fopenmp -std=c99 (oracle port)")
+    return {"value": per_build[best], "unit": "halo·px/s", "cores": threads,
+            "kind": "reference" if builds else "port",
+            "sample": f"gradient2_grid_CPU (the reference's CPU Hessian grid) of all {lens.n} halos on {rows} full "
+                      f"rows (4096 px) of the same grid from row 2048, median of {runs} runs after one warm-up",
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(), "omp_threads": threads, "compiler_flags": flags,
+            "per_build_halo_px_per_s": per_build}
 
 
 def bench_bayes(a):
@@ -247,23 +297,32 @@ def bench_chi(a):
             "overflow_samples": int((status != 0).sum()), "cpu_baseline": cpu}
 
 
-def bench_serve(a):
-    """Map-serving loop: Poisson arrivals at each rate of --rates; grid n = 2^U[8,13] rounded to a
-    multiple of 64; a pool of 32 resident models drawn like the cluster lens with
-    H = floor(10^U[1, log10 2000]) halos; map kind uniform over {gradient (dpl), ampli (mu), shear,
-    mass (Kmap), pot}; precision uniform over {FP32C, FP64}. Reports per rate the throughput and the
-    p50/p99 latency from submission to result ready on the device and to result copied to the host."""
-    import threading
-    rng = np.random.default_rng(9)
+def serve_pool(n_models=32, seed=9):
+    """Resident models of the serving workload: cluster-like lenses with H = floor(10^U[1, log10 2000])
+    halos (3 cluster-scale dPIE + luminosity-scaled galaxies)."""
+    rng = np.random.default_rng(seed)
     pool = []
-    for k in range(32):
+    for k in range(n_models):
         H = int(10 ** rng.uniform(1.0, np.log10(2000.0)))
         L = M.cluster_lens(seed=1000 + k, ngal=max(H - 3, 0))
         L.dlsds[:] = 0.7
         L._soa = None
         pool.append(L)
+    return pool
+
+
+def serve_sweep(pool, rates, requests, slots=8, budget_gb=64, seed=9, until_saturation=False):
+    """Poisson arrivals at each rate (req/s): grid n = 2^U[8,13] rounded to a multiple of 64; map kind
+    uniform over {gradient (dpl), ampli (mu), shear, mass (Kmap), pot}; precision uniform over {FP32C,
+    FP64}. A producer thread submits at the arrival times, the consumer
+    fetches in submission order. Per rate: offered and achieved request rate, throughput (halo·px/s)
+    and p50/p99 latency from submission to result ready on the device (and to result on the host).
+    until_saturation: keep doubling the last rate until the achieved rate falls below 90 % of the
+    offered one."""
+    import threading
+    rng = np.random.default_rng(seed)
     r = lthip.cosmo_dlsds(0.4, 2.0, M.flat_lcdm(70.0, 0.3))
-    srv = lthip.Server(nslots=a.slots, hbm_budget_bytes=a.budget_gb << 30)
+    srv = lthip.Server(nslots=slots, hbm_budget_bytes=budget_gb << 30)
     mids = [srv.add_model(L) for L in pool]
     kinds = [(lthip.MAP_DPL, 1), (lthip.MAP_AMPLI, 1), (lthip.MAP_SHEAR, 1), (lthip.MAP_MASS, 1), (lthip.MAP_POT, 1)]
 
@@ -273,11 +332,14 @@ def bench_serve(a):
         k, m = kinds[rng.integers(len(kinds))]
         mi = int(rng.integers(len(pool)))
         grid = M.Grid(M.grid_param(-150.0, 150.0, -150.0, 150.0), n, n)
-        return mi, k, m, grid, int(rng.integers(2))
+        p = int(rng.integers(2))
+        return mi, k, m, grid, p if k != lthip.MAP_POT else 0
 
     rows = []
-    for rate in a.rates:
-        reqs = [make_req() for _ in range(a.requests)]
+    todo = list(rates)
+    while todo:
+        rate = todo.pop(0)
+        reqs = [make_req() for _ in range(requests)]
         gaps = rng.exponential(1.0 / rate, len(reqs))
         srv.reset_stats()
         tickets = [None] * len(reqs)
@@ -303,12 +365,21 @@ def bench_serve(a):
         th.join()
         wall = time.perf_counter() - t0
         st = srv.stats()
-        rows.append({"rate_per_s": rate, "requests": len(reqs), "wall_s": wall,
-                     "halo_px_per_s": st["halo_px"] / wall, "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
-                     "p50_fetch_ms": st["p50_fetch_ms"], "p99_fetch_ms": st["p99_fetch_ms"],
-                     "bytes_peak": st["bytes_peak"]})
-        print(f"[serve] rate {rate}/s: {json.dumps(rows[-1])}", file=sys.stderr, flush=True)
+        row = {"offered_req_per_s": rate, "achieved_req_per_s": len(reqs) / wall, "requests": len(reqs),
+               "wall_s": wall, "halo_px_per_s": st["halo_px"] / wall, "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
+               "p50_fetch_ms": st["p50_fetch_ms"], "p99_fetch_ms": st["p99_fetch_ms"], "bytes_peak": st["bytes_peak"]}
+        rows.append(row)
+        print(f"[serve] {json.dumps(row)}", file=sys.stderr, flush=True)
+        if until_saturation and not todo and row["achieved_req_per_s"] >= 0.9 * rate:
+            todo.append(rate * 2)
     srv.close()
+    return rows
+
+
+def bench_serve(a):
+    """Map-serving loop under Poisson load, rates swept (--rates; --until-saturation doubles the last
+    rate until the achieved rate falls below 90 % of the offered one), --requests per rate."""
+    rows = serve_sweep(serve_pool(), a.rates, a.requests, a.slots, a.budget_gb, until_saturation=a.until_saturation)
     best = max(rows, key=lambda x: x["halo_px_per_s"])
     return {"metric": "map-serving throughput (halo·px/s) and p50/p99 latency under Poisson load",
             "value": best["halo_px_per_s"], "unit": "halo·px/s", "n_gpus": 1, "steps": a.requests,
@@ -323,8 +394,11 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", choices=("cluster", "bayes", "chi", "serve"), default="cluster")
     ap.add_argument("--samples", type=int, default=0, help="bayes: 256, chi: 1024 by default")
-    ap.add_argument("--rates", type=float, nargs="*", default=[2.0, 8.0, 32.0])
-    ap.add_argument("--requests", type=int, default=40)
+    ap.add_argument("--rates", type=float, nargs="*", default=[8.0, 16.0, 32.0, 64.0])
+    ap.add_argument("--requests", type=int, default=500)
+    ap.add_argument("--until-saturation", action="store_true")
+    ap.add_argument("--no-workloads", action="store_true",
+                    help="cluster workload only: skip the short Bayes, chi^2 and serving sub-results")
     ap.add_argument("--slots", type=int, default=8)
     ap.add_argument("--budget-gb", type=int, default=64)
     ap.add_argument("--gpus", type=int, default=1)
@@ -373,9 +447,29 @@ def main():
     value = a.steps * halo_px / (total_ms * 1e-3)
     avg_ms = float(np.mean(per))
     flops_per_map = lens.halo_flops("hessian") * grid.npix
-    achieved = flops_per_map / (avg_ms * 1e-3) / 1e12
+    algorithmic = flops_per_map / (avg_ms * 1e-3) / 1e12
     peak = PEAK_FP64_TFLOPS if prec == lthip.PREC_FP64 else PEAK_FP32_TFLOPS
     traffic = read_traffic() if prec == lthip.PREC_FP64 else None
+    counters = read_counters() if prec == lthip.PREC_FP64 else None
+    rate_dev = halo_px / (avg_ms * 1e-3)  # halo x px per second of kernel time
+    if counters:
+        achieved = counters["fp64_hw_flop_per_halo_px"] * rate_dev / 1e12
+        roofline = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+                    "traffic": traffic,
+                    "flops": "hardware FP64 flops, 64 lanes x (ADD + MUL + 2 FMA) per wavefront instruction, "
+                             f"{counters['fp64_hw_flop_per_halo_px']:.2f} per dPIE halo x px (rocprofv3 "
+                             "SQ_INSTS_VALU_{ADD,MUL,FMA}_F64, profiles/hessian_fields_4096_503halos_valu_counters.md)",
+                    "valu_issue_frac": counters["valu_issue_slots_per_halo_px"] * 4 * rate_dev / (SIMDS * CLOCK_HZ),
+                    "valu_issue": f"SQ_ACTIVE_INST_VALU slots ({counters['valu_issue_slots_per_halo_px']:.3f} per "
+                                  "dPIE halo x px) x 4 cycles / (1024 SIMDs x 2.4 GHz x time)",
+                    "algorithmic": {"achieved": algorithmic, "frac": algorithmic / peak,
+                                    "convention": "262 flop per dPIE Hessian halo*px (division, sqrt = 8 each); "
+                                                  "this kernel has no division or sqrt, so the convention "
+                                                  "overstates its work and can exceed the peak"}}
+    else:
+        roofline = {"bound": "valu", "achieved": algorithmic, "peak": peak, "unit": "TFLOP/s",
+                    "frac": algorithmic / peak, "traffic": traffic,
+                    "flops": "algorithmic convention: 262 flop per dPIE Hessian halo*px (division, sqrt = 8 each)"}
 
     res = {
         "metric": METRIC,
@@ -399,15 +493,7 @@ def main():
         "latency_ms": {"p50": percentile(per, 50), "p99": percentile(per, 99), "mean": avg_ms},
         "autotune": tuned,
         "wall_s": wall,
-        "roofline": {
-            "bound": "valu",
-            "achieved": achieved,
-            "peak": peak,
-            "unit": "TFLOP/s",
-            "frac": achieved / peak,
-            "traffic": traffic,
-            "flops_convention": "algorithmic: 262 flop per dPIE Hessian halo*px (div, sqrt = 8)",
-        },
+        "roofline": roofline,
     }
     if not a.no_secondary:
         # the other precision on the same steps, then the FP32C error against FP64 (untimed)
@@ -429,6 +515,17 @@ def main():
                      "fp32c_over_fp64": (v2 / value) if name == "fp32c" else (value / v2),
                      "fp32c_max_abs_err_vs_fp64": errs,
                      "fp32c_max_rel_err_vs_fp64_where_abs_detA_ge_1e-3": rel}
+        # the FP32C map with the lanes-over-halos launch shape (not among the autotuner's shapes: its sums
+        # run in another order)
+        per3, total3, _ = run_maps(model, outs, grid, r, lthip.PREC_FP32C, a.steps, a.warmup, stream,
+                                   lthip.PX_HALO_LANES)
+        hl = [o.download() for o in outs[:4]]
+        res["fp32c_halo_lanes"] = {
+            "value": a.steps * halo_px / (total3 * 1e-3), "unit": "halo·px/s", "ms_per_step": total3 / a.steps,
+            "latency_ms": {"p50": percentile(per3, 50), "p99": percentile(per3, 99)},
+            "over_fp64": (a.steps * halo_px / (total3 * 1e-3)) / (value if prec == lthip.PREC_FP64 else v2),
+            "max_abs_err_vs_fp64": {n: float(np.abs(x - y).max())
+                                    for n, x, y in zip(("kappa", "gamma1", "gamma2", "detA"), hl, f64)}}
         # gradient maps of the same lens and grid in both precisions, each autotuned first
         gm = lthip.Model(lens, lthip.MAP_GRADIENT)
         gsteps = min(a.steps, 10)
@@ -445,6 +542,22 @@ def main():
         res["gradient_fp32c"]["max_abs_err_vs_fp64_arcsec"] = float(
             np.hypot(grads["fp32c"][0] - grads["fp64"][0], grads["fp32c"][1] - grads["fp64"][1]).max())
         res["gradient_fp32c"]["fp32c_over_fp64"] = res["gradient_fp32c"]["value"] / res["gradient_fp64"]["value"]
+        # potential maps (mode 3, no scaling) of the same lens and grid in both precisions
+        pm = lthip.Model(lens, lthip.MAP_POT)
+        psteps = min(a.steps, 3)
+        pots = {}
+        for pname, pprec in (("fp64", lthip.PREC_FP64), ("fp32c", lthip.PREC_FP32C)):
+            pm.autotune(lthip.MAP_POT, 3, grid, pprec, stream)
+            perp, totalp, _ = run_maps(pm, outs[:1], grid, 1.0, pprec, psteps, 1, stream, 0, lthip.MAP_POT, mode=3)
+            pots[pname] = outs[0].download()
+            res["potential_" + pname] = {
+                "value": psteps * halo_px / (totalp * 1e-3), "unit": "halo·px/s", "ms_per_step": totalp / psteps,
+                "steps": psteps, "latency_ms": {"p50": percentile(perp, 50), "p99": percentile(perp, 99)}}
+        pm.close()
+        dphi = np.abs(pots["fp32c"] - pots["fp64"])
+        res["potential_fp32c"]["max_abs_err_vs_fp64_arcsec2"] = float(dphi.max())
+        res["potential_fp32c"]["max_rel_err_vs_fp64"] = float((dphi / np.maximum(np.abs(pots["fp64"]), 1e-300)).max())
+        res["potential_fp32c"]["fp32c_over_fp64"] = res["potential_fp32c"]["value"] / res["potential_fp64"]["value"]
     if a.copy_bytes > 0:
         src = lthip.DeviceArray(a.copy_bytes, np.uint8)
         dst = lthip.DeviceArray(a.copy_bytes, np.uint8)
@@ -457,6 +570,25 @@ def main():
         res["copy_calibration"] = {"bytes": a.copy_bytes, "ms": ms, "GB/s": 2 * a.copy_bytes / ms / 1e6}
     if not a.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(lens, grid)
+    if not a.no_workloads:
+        # short runs of the other workloads, so that every bench line measures them too
+        for o in outs:
+            o.close()
+        model.close()
+        sub = argparse.Namespace(samples=256, steps=5, warmup=1, no_cpu_baseline=True)
+        b = bench_bayes(sub)
+        res["bayes_maps"] = {"workload": b["config"]["workload"], "steps": sub.steps,
+                             "fp64": b["fp64"], "fp32c": b["fp32c"],
+                             "fp32c_over_fp64": b["fp32c"]["value"] / b["fp64"]["value"]}
+        sub = argparse.Namespace(samples=1024, steps=3, warmup=1, no_cpu_baseline=True)
+        c = bench_chi(sub)
+        res["chi"] = {"workload": c["config"]["workload"], "steps": sub.steps, "value": c["value"],
+                      "unit": c["unit"], "ms_per_step": c["ms_per_step"], "chi_p50": c["chi_p50"],
+                      "found_mean": c["found_mean"], "overflow_samples": c["overflow_samples"]}
+        rows = serve_sweep(serve_pool(), [20.0, 80.0], 120)
+        res["serving"] = {"workload": "Poisson map requests, n in 256..8192, H in 10..2000, 5 map kinds, FP32C or "
+                                      "FP64, 8 slots, 64 GB arena (short sweep; the full one is bench.py "
+                                      "--workload serve)", "sweep": rows}
     print(json.dumps(res), flush=True)
 
 

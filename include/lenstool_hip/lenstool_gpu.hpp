@@ -33,6 +33,36 @@ void gradient2_grid_GPU(type_t* grid_grad2_a, type_t* grid_grad2_b, type_t* grid
                         const struct grid_param* frame, const struct Potential_SOA* lens, int nhalos, type_t dx,
                         type_t dy, int nbgridcells_x, int nbgridcells_y, int istart = 0, int jstart = 0);
 
+// ---- the launchers on device data (src/grid_gradient_GPU.cuh:40-41, src/grid_gradient2_GPU.cuh:36-43) --
+// Output planes, frame and the Potential_SOA descriptor are device pointers; the descriptor's arrays
+// are device arrays (the subset type, position_x/y, b0, ellipticity_angle, ellipticity_potential,
+// rcore, rcut, anglecos/sin, and dlsds for the Kmap). Synchronous.
+void module_potentialDerivatives_totalGradient_SOA_CPU_GPU(type_t* grid_grad_x, type_t* grid_grad_y,
+                                                           const struct grid_param* frame,
+                                                           const struct Potential_SOA* lens_gpu, int nbgridcells,
+                                                           int nhalos);
+void module_potentialDerivatives_totalGradient_SOA_CPU_GPU(type_t* grid_grad_x, type_t* grid_grad_y,
+                                                           const struct grid_param* frame,
+                                                           const struct Potential_SOA* lens, int nhalos, type_t dx,
+                                                           type_t dy, int nbgridcells_x, int nbgridcells_y,
+                                                           int istart, int jstart);
+void module_potentialDerivatives_totalGradient2_SOA_CPU_GPU(type_t* grid_grad2_a, type_t* grid_grad2_b,
+                                                            type_t* grid_grad2_c, type_t* grid_grad2_d,
+                                                            const struct grid_param* frame,
+                                                            const struct Potential_SOA* lens_gpu, int nbgridcells,
+                                                            int nhalos);
+void module_potentialDerivatives_totalGradient2_SOA_CPU_GPU(type_t* grid_grad2_a, type_t* grid_grad_b,
+                                                            type_t* grid_grad2_c, type_t* grid_grad_d,
+                                                            const struct grid_param* frame,
+                                                            const struct Potential_SOA* lens, int nhalos, type_t dx,
+                                                            type_t dy, int nbgridcells_x, int nbgridcells_y,
+                                                            int istart, int jstart);
+void module_potentialDerivatives_Kmap_SOA_CPU_GPU(type_t* grid_grad2_a, type_t* grid_grad2_b, type_t* grid_grad2_c,
+                                                  type_t* grid_grad2_d, const struct grid_param* frame,
+                                                  const struct Potential_SOA* lens_gpu, int nhalos, type_t dx,
+                                                  type_t dy, int nbgridcells_x, int nbgridcells_y, int istart,
+                                                  int jstart);
+
 // ---- src/grid_potential_GPU.cuh:31-32 -----------------------------------------------------------
 void potential_grid_GPU(type_t* grid_potential, const struct grid_param* frame, const struct Potential_SOA* lens,
                         int nhalos, type_t dx, type_t dy, int nbgridcells_x, int nbgridcells_y, int istart,
@@ -51,6 +81,8 @@ map_gpu_function_t select_map_ampli_function(const struct runmode_param* runmode
 map_gpu_function_t select_map_shear_function(const struct runmode_param* runmode);
 map_gpu_function_t select_map_mass_function(const struct runmode_param* runmode);
 map_pot_function_t select_map_potential_function(const struct runmode_param* runmode);
+// src/grid_map_dpl_GPU.cuh:39 (mode 1: dpl_1_grid_CPU_GPU, the in-place D_ls/D_s scaling)
+map_gpu_function_t select_map_dpl_function(const struct runmode_param* runmode);
 
 void amplif_1_grid_CPU_GPU(type_t*, type_t*, type_t*, type_t*, type_t*, type_t, type_t, type_t, type_t, type_t, int,
                            int, const struct grid_param*);
@@ -83,6 +115,8 @@ void mass_4_grid_CPU_GPU(type_t*, type_t*, type_t*, type_t*, type_t*, type_t, ty
 void potential_1_grid_CPU_GPU(type_t*, type_t, type_t, type_t, type_t, type_t, int, int, const struct grid_param*);
 void potential_2_grid_CPU_GPU(type_t*, type_t, type_t, type_t, type_t, type_t, int, int, const struct grid_param*);
 void potential_3_grid_CPU_GPU(type_t*, type_t, type_t, type_t, type_t, type_t, int, int, const struct grid_param*);
+void dpl_1_grid_CPU_GPU(type_t*, type_t*, type_t*, type_t*, type_t*, type_t, type_t, type_t, type_t, type_t, int, int,
+                        const struct grid_param*);
 void dpl_grid_CPU_GPU(type_t* map, type_t dl0s, type_t ds, type_t dl, type_t h, type_t z, int nbgridcells_x,
                       int nbgridcells_y, const struct grid_param* frame);
 

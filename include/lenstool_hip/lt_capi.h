@@ -15,9 +15,9 @@
  *  2. Resident models and device-pointer entry points: a lens model is packed and uploaded once
  *     (lt_model_create) and maps are computed asynchronously on a caller stream into device memory.
  *
- * precision: LT_PREC_FP64 or LT_PREC_FP32C (lt_status.h). FP32C (FP32 halo terms with FP64 offsets,
- * FP64 terms next to the cores and FP64 chunk sums) applies to the Hessian-derived maps and to the
- * gradient / displacement maps; potential maps are FP64 only.
+ * precision: LT_PREC_FP64 or LT_PREC_FP32C (lt_status.h). FP32C (FP32 halo terms with FP64 offsets
+ * and FP64 chunk sums; FP64 terms next to the cores for the Hessian, gradient and displacement) applies
+ * to every map kind.
  */
 #ifndef LENSTOOL_HIP_LT_CAPI_H
 #define LENSTOOL_HIP_LT_CAPI_H
@@ -41,6 +41,7 @@ int lt_dev_alloc(void** ptr, long long bytes);
 int lt_dev_free(void* ptr);
 int lt_memcpy_h2d(void* dst, const void* src, long long bytes, void* stream);  /* waits when stream == NULL */
 int lt_memcpy_d2h(void* dst, const void* src, long long bytes, void* stream);
+/* bytes of dst set to (unsigned char)value by a fill kernel (no memset node inside a captured graph) */
 int lt_memset_d(void* dst, int value, long long bytes, void* stream);
 /* device-to-device copy done by a kernel (a streaming copy of known size, also used to calibrate
  * hardware traffic counters) */
@@ -77,7 +78,8 @@ int lt_kmap_grid(double* a, double* b, double* c, double* d, const lt_grid_param
                  int nbgridcells_x, int nbgridcells_y, int istart, int jstart, int precision);
 
 /* Lensing potential. Replaces potential_grid_GPU(..., dx, dy, nx, ny, istart, jstart)
- * (src/grid_potential_GPU.cuh:31); types 5, 8 and 81 (the reference has 81 only). FP64. */
+ * (src/grid_potential_GPU.cuh:31); types 5, 8 and 81 (the reference has 81 only). FP64 (lt_map_device
+ * with LT_MAP_POT also has an FP32C kernel). */
 int lt_potential_grid(double* grid_potential, const lt_grid_param* frame, const lt_potential_soa* lens,
                       int nhalos, double dx, double dy, int nbgridcells_x, int nbgridcells_y,
                       int istart, int jstart);
@@ -154,7 +156,11 @@ int lt_map_factors(int kind, int mode, const lt_cosmo_param* cosmo, double z_len
 /* Compute one map into device arrays on `stream` (asynchronous). kind/mode as lt_map_grid plus
  * LT_MAP_HESSIAN (out0..out3 = a, b, c, d), LT_MAP_GRADIENT (out0, out1 = x, y) and LT_MAP_FIELDS
  * (out0..out4 = kappa, gamma1, gamma2, det A, mu). r = D_ls/D_s used by the map formulas, mult = the
- * mass / potential multiplier of the mode. px = pixels per thread (0 = the tuned default). */
+ * mass / potential multiplier of the mode. px = launch shape: 1, 2 or 4 pixels per thread (results
+ * identical bit for bit), 0 = the tuned default, or LT_PX_HALO_LANES (FP32C Hessian-derived maps
+ * only): lanes over halos, a wavefront per 16-pixel row segment; its FP32 terms are summed in
+ * another order, so its results differ from the other shapes' within the FP32C error. */
+#define LT_PX_HALO_LANES 16
 int lt_map_device(lt_model model, int kind, int mode, double* out0, double* out1, double* out2, double* out3,
                   double* out4, double r, double mult, const lt_grid_desc* grid, int precision, int px,
                   void* stream);
@@ -162,7 +168,7 @@ int lt_map_device(lt_model model, int kind, int mode, double* out0, double* out1
 /* Time the map with 1, 2 and 4 pixels per thread (one warm-up and two timed launches each, into
  * scratch planes) and make the fastest the default (px = 0) for maps of this family, precision,
  * halo-count octave and pixel-count octave in this process. ms[3] (optional) gets the per-launch
- * times. Results do not depend on px: only speed does. */
+ * times. Results do not depend on these shapes: only speed does. */
 int lt_autotune(lt_model model, int kind, int mode, const lt_grid_desc* grid, int precision, void* stream,
                 int* best_px, float* ms);
 
@@ -178,10 +184,14 @@ int lt_graph_destroy(lt_graph graph);
 
 /* ---- map-serving loop ---------------------------------------------------------------------------
  * The reference has no counterpart (its map drivers, utils/exec/main.cpp:397-545, run one
- * synchronous map after another). A server keeps lens models resident on the calling thread's GPU,
- * runs requests on nslots streams and admits queued requests as slots and HBM (hbm_budget_bytes,
- * reserved at creation; the output planes of admitted requests are carved out of it) become free;
- * work of different requests overlaps on the device. Results stay in device memory until fetched. */
+ * synchronous map after another). A server keeps lens models resident on the calling thread's GPU
+ * and admits queued requests as HBM frees up (hbm_budget_bytes, reserved at creation; the output
+ * planes of admitted requests are carved out of it and held until fetched). Admitted requests are
+ * cut into row tiles, which the nslots streams take in turn from every admitted request, so work
+ * of different requests overlaps on the device and a small request does not wait for the whole of
+ * a large one. A later request is admitted ahead of an earlier one that does not fit only when that
+ * cannot keep the earlier one out for good: a caller that fetches its results in submission order
+ * never waits forever. Results stay in device memory until fetched. */
 typedef struct lt_server_s* lt_server;
 typedef struct lt_request {
     int model;            /* id from lt_server_add_model                                          */
@@ -209,10 +219,13 @@ int lt_server_create(lt_server* out, int nslots, long long hbm_budget_bytes);
 int lt_server_add_model(lt_server server, const lt_potential_soa* lens, int nhalos, int* model_id);
 /* Queue a request (non-blocking). LT_ERR_CAPACITY if its planes alone exceed the budget. */
 int lt_server_submit(lt_server server, const lt_request* req, long long* ticket);
-/* Wait for a request (timeout_ms < 0: forever); LT_ERR_BUSY on timeout, the request's status if it failed. */
+/* Wait for a request (timeout_ms < 0: no time limit); the request's status if it failed. LT_ERR_BUSY
+ * on timeout, and also, whatever the timeout, when the request cannot be admitted until results
+ * submitted before it are fetched and no work or fetch is under way (the wait would never end). */
 int lt_server_wait(lt_server server, long long ticket, double timeout_ms);
 /* Wait, copy the request's planes to the given host arrays (NULL = skip a plane), release the
- * ticket and its device memory. Plane order as lt_map_device. */
+ * ticket and its device memory. Plane order as lt_map_device. LT_ERR_BUSY as lt_server_wait (the
+ * ticket stays valid). */
 int lt_server_fetch(lt_server server, long long ticket, double* out0, double* out1, double* out2, double* out3,
                     double* out4, lt_request_times* times);
 int lt_server_get_stats(lt_server server, lt_server_stats* stats);
@@ -299,7 +312,9 @@ int lt_critic_device(const double* det, const double* kappa, const lt_grid_desc*
  * and kappa from the fields map in `precision`, then lt_critic_device), and optionally the
  * caustics: each segment end theta mapped to beta = theta - (D_ls/D_s) alpha(theta), alpha the
  * FP64 deflection at that point (4 doubles per segment, like `critic`). Host arrays; any of
- * critic, caustic, kinds may be NULL. */
+ * critic, caustic, kinds may be NULL. A model with external-shear (type 14) halos has no deflection
+ * kernel: critic, kinds and *nsegments are filled, caustic is set to NaN and the call returns
+ * LT_ERR_TYPE. */
 int lt_critic_lines(const lt_cosmo_param* cosmo, const lt_grid_param* frame, const lt_potential_soa* lens, int nhalos,
                     double z_lens, double z_source, int nx, int ny, int precision, double* critic, double* caustic,
                     int* kinds, long long max_segments, long long* nsegments);

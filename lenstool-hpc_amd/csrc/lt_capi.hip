@@ -115,8 +115,8 @@ int octave(long long v)
 TuneKey tune_key(int kind, int precision, int nhalos, long long npix)
 {
     const int fam = (kind == LT_MAP_GRADIENT || kind == LT_MAP_DPL) ? 0 : kind == LT_MAP_POT ? 2 : 1;
-    const int prec = fam == 1 ? precision : LT_PREC_FP64;  // FP32C exists for the Hessian family only
-    return TuneKey{fam, prec, octave(nhalos), octave(npix)};
+    // every family has an FP64 and an FP32C kernel, each tuned on its own
+    return TuneKey{fam, precision, octave(nhalos), octave(npix)};
 }
 
 int default_px(int kind, int precision, int nhalos, long long npix)
@@ -148,7 +148,11 @@ int launch_map(const DevModel& m, int kind, int mode, double* const outs[5], dou
     o.r = r;
     o.mult = mult;
     if (px <= 0) px = default_px(kind, precision, nhalos, (long long)g.nx * g.ny);
-    if (px != 1 && px != 2 && px != 4) return fail(LT_ERR_INVALID, "px must be 0, 1, 2 or 4");
+    const bool hessian_family = kind != LT_MAP_GRADIENT && kind != LT_MAP_DPL && kind != LT_MAP_POT;
+    if (px == LT_PX_HALO_LANES && !(hessian_family && precision == LT_PREC_FP32C))
+        return fail(LT_ERR_INVALID, "px = LT_PX_HALO_LANES applies to FP32C Hessian-derived maps only");
+    if (px != 1 && px != 2 && px != 4 && px != LT_PX_HALO_LANES)
+        return fail(LT_ERR_INVALID, "px must be 0, 1, 2, 4 or LT_PX_HALO_LANES");
     if (precision != LT_PREC_FP64 && precision != LT_PREC_FP32C)
         return fail(LT_ERR_INVALID, "precision must be LT_PREC_FP64 or LT_PREC_FP32C");
     hipError_t e;
@@ -159,10 +163,11 @@ int launch_map(const DevModel& m, int kind, int mode, double* const outs[5], dou
         else e = launch_gradient64(m.recs, m.runs, m.nruns, g, o, px, s);
         break;
     case LT_MAP_POT:
-        e = launch_potential64(m.recs, m.runs, m.nruns, g, o, s);
+        if (precision == LT_PREC_FP32C) e = launch_potential32(m.recs, m.recs32, m.runs, m.nruns, g, o, px, s);
+        else e = launch_potential64(m.recs, m.runs, m.nruns, g, o, px, s);
         break;
     default:
-        // FP32C covers the Hessian-derived and the gradient maps; potential maps are FP64 either way
+        // FP32C covers the Hessian-derived and the gradient maps
         if (precision == LT_PREC_FP32C) e = launch_hessian32(m.recs, m.recs32, m.runs, m.nruns, g, o, px, s);
         else e = launch_hessian64(m.recs, m.runs, m.nruns, g, o, px, s);
         break;
@@ -358,7 +363,9 @@ int lt_memset_d(void* dst, int value, long long bytes, void* stream)
 {
     if (bytes < 0 || (bytes && !dst)) return fail(LT_ERR_INVALID, "bad memset");
     if (!bytes) return LT_OK;
-    HIP_TRY(hipMemsetAsync(dst, value, (size_t)bytes, (hipStream_t)stream));
+    // a fill kernel rather than hipMemsetAsync, so that a captured graph holds no memset node
+    hipError_t e = launch_fill_bytes(dst, (unsigned char)value, (size_t)bytes, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "fill kernel");
     if (!stream) HIP_TRY(hipStreamSynchronize(nullptr));
     return LT_OK;
 }
@@ -527,8 +534,6 @@ int lt_map_grid(int kind, int mode, double* map0, double* map1, const lt_cosmo_p
     double* outs[5] = {map0, kind == LT_MAP_DPL ? map1 : nullptr, nullptr, nullptr, nullptr};
     Use use = kind == LT_MAP_DPL ? Use::Gradient : kind == LT_MAP_POT ? Use::Potential
               : kind == LT_MAP_MASS ? Use::Kmap : Use::Hessian;
-    if (kind == LT_MAP_POT && precision != LT_PREC_FP64)
-        return fail(LT_ERR_INVALID, "the potential map is computed in FP64 only");
     return host_map(kind, mode, use, outs, frame, lens, nhalos, r, mult, dx, dy, nx, ny, istart, jstart, precision);
 }
 
@@ -851,14 +856,17 @@ int lt_autotune(lt_model m, int kind, int mode, const lt_grid_desc* grid, int pr
     for (int k = 0; k < np; ++k) outs[k] = buf + k * plane;
     lt_grid_desc g = *grid;
     g.pitch = g.nx;
+    // the shapes whose results are bit-identical to each other (the halo-lane shape sums the FP32 terms
+    // in another order: it is used when asked for explicitly)
     const int cand[3] = {1, 2, 4};
+    const int ncand = 3;
     float best = 1e30f;
     int bpx = 2;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipError_t he = hipEventCreate(&e0);
     if (he == hipSuccess) he = hipEventCreate(&e1);
-    for (int i = 0; i < 3 && he == hipSuccess && st == LT_OK; ++i) {
-        const int px = (kind == LT_MAP_POT) ? 1 : cand[i];
+    for (int i = 0; i < ncand && he == hipSuccess && st == LT_OK; ++i) {
+        const int px = cand[i];
         st = lt_map_device(m, kind, mode, outs[0], outs[1], outs[2], outs[3], outs[4], 0.7, 1.0, &g, precision, px,
                            stream);  // warm
         if (st == LT_OK) he = hipEventRecord(e0, s);

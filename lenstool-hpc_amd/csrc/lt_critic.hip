@@ -319,6 +319,10 @@ int lt_critic_lines(const lt_cosmo_param* cosmo, const lt_grid_param* frame, con
     if ((critic || caustic || kinds) && max_segments < 0) return fail(LT_ERR_INVALID, "negative max_segments");
     double r = 0, mult = 1;
     const double dx = (frame->xmax - frame->xmin) / (nx - 1), dy = (frame->ymax - frame->ymin) / (ny - 1);
+    // external-shear (type 14) halos have a Hessian but no deflection kernel (the reference has none
+    // either): their critical lines are computed, their caustics are not
+    bool no_deflection = false;
+    for (int i = 0; i < nhalos && lens->type; ++i) no_deflection |= lens->type[i] == 14;
     int st = lt_map_factors(LT_MAP_AMPLI, 1, cosmo, z_lens, z_source, dx, dy, &r, &mult);
     if (st) return st;
     lt_model m = nullptr;
@@ -352,7 +356,9 @@ int lt_critic_lines(const lt_cosmo_param* cosmo, const lt_grid_param* frame, con
             if ((st = lt_memcpy_d2h(segs.data(), dsegs, got * 4 * (long long)sizeof(double), nullptr))) break;
             if (kinds && (st = lt_memcpy_d2h(kinds, dkinds, got * (long long)sizeof(int), nullptr))) break;
             if (critic) std::copy(segs.begin(), segs.begin() + got * 4, critic);
-            if (caustic) {
+            if (caustic && no_deflection) {
+                std::fill(caustic, caustic + got * 4, std::nan(""));
+            } else if (caustic) {
                 // beta = theta - r * alpha(theta) at both ends of every segment
                 std::vector<double> px((size_t)got * 2), py((size_t)got * 2), ax((size_t)got * 2), ay((size_t)got * 2);
                 for (long long k = 0; k < got; ++k) {
@@ -378,6 +384,9 @@ int lt_critic_lines(const lt_cosmo_param* cosmo, const lt_grid_param* frame, con
     lt_model_destroy(m);
     if (st) return st;
     *nsegments = total;
+    if (st2 == LT_OK && caustic && no_deflection)
+        return fail(LT_ERR_TYPE, "critical lines written; caustics unavailable (NaN): the model has external-shear "
+                                 "(type 14) halos, which have no deflection kernel");
     return st2;
 }
 

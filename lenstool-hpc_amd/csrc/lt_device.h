@@ -91,6 +91,21 @@ struct alignas(64) GradRec32 {
 };
 static_assert(sizeof(GradRec32) == sizeof(HaloRec32), "gradient fp32 record is 128 bytes");
 
+// Single-precision record of a halo for the FP32-compensated potential (models packed for potential
+// use hold these in place of HaloRec32, same size): the constants of the algebraic I_{0.5} potential
+// of lt_terms64.h pot_halo in FP32.
+struct alignas(64) PotRec32 {
+    float cs, sn;                       // 0, 1    cos, sin of the angle
+    float kx, ky;                       // 2, 3    1/(1+e)^2, 1/(1-e)^2
+    float pkp, pkq;                     // 4, 5    2 sqrt(e)/(1-e), 2 sqrt(e)/(1+e)
+    float ra, ira, ra2;                 // 6..8    rcore, 1/rcore, rcore^2
+    float rb, irb, rb2;                 // 9..11   rcut, 1/rcut, rcut^2
+    float amp;                          // 12      PIEMD / dPIE amplitude (ampg); SIS b0
+    float sa, sb;                       // 13, 14  SIS: 1 - e, 1 + e
+    float pad[17];                      // 15..31
+};
+static_assert(sizeof(PotRec32) == sizeof(HaloRec32), "potential fp32 record is 128 bytes");
+
 // Grid description (pixel (col, row) of the output sits at (xmin + (col+istart) dx,
 // ymin + (row+jstart) dy); output index row*pitch + col).
 struct GridDesc {

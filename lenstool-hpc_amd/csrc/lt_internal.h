@@ -15,6 +15,9 @@ int hip_error(hipError_t e, const char* where);
 int current_device(int* dev);
 // fill n ints of device memory with v (a kernel, so it can sit in a captured graph)
 hipError_t launch_fill_i32(int32_t* p, int32_t v, size_t n, hipStream_t s);
+// fill `bytes` bytes of device memory with v (kernels only: no memset node when captured)
+hipError_t launch_fill_bytes(void* dst, unsigned char v, size_t bytes, hipStream_t s);
+
 
 }  // namespace lt
 

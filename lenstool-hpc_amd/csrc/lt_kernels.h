@@ -17,7 +17,7 @@ hipError_t launch_hessian64(const HaloRec* halos, const Run* runs, int nruns, co
 hipError_t launch_gradient64(const HaloRec* halos, const Run* runs, int nruns, const GridDesc& g,
                              const OutDesc& o, int px, hipStream_t s);
 hipError_t launch_potential64(const HaloRec* halos, const Run* runs, int nruns, const GridDesc& g,
-                              const OutDesc& o, hipStream_t s);
+                              const OutDesc& o, int px, hipStream_t s);
 // gradient at npts points for nsamples lens models (records of sample s start at s*halo_stride);
 // outputs [nsamples*npts]
 hipError_t launch_points_gradient64(const HaloRec* halos, const Run* runs, int nruns, const double* px,
@@ -33,6 +33,10 @@ hipError_t launch_hessian32(const HaloRec* halos64, const HaloRec32* halos32, co
 // gradient / displacement maps; the model must be packed for gradient use (GradRec32 records)
 hipError_t launch_gradient32(const HaloRec* halos64, const HaloRec32* recs32, const Run* runs, int nruns,
                              const GridDesc& g, const OutDesc& o, int px, hipStream_t s);
+
+// potential maps; the model must be packed for potential use (PotRec32 records)
+hipError_t launch_potential32(const HaloRec* halos64, const HaloRec32* recs32, const Run* runs, int nruns,
+                              const GridDesc& g, const OutDesc& o, int px, hipStream_t s);
 
 // Streaming device copy (16-byte vectors plus a byte tail).
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);

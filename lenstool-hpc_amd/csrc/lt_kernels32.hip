@@ -26,6 +26,7 @@
 // dv/dy = 2 s a b, and use |D|^2 / S = (w - v) / a^2, 2 s (w - v) + cx1 (2 r s - y) = t.)
 #include <hip/hip_runtime.h>
 
+#include "lenstool_hip/lt_capi.h"
 #include "lenstool_hip/lt_status.h"
 #include "lt_device.h"
 #include "lt_kernels.h"
@@ -364,12 +365,207 @@ static hipError_t launch_hessian32_px(const HaloRec* h64, const HaloRec32* h32, 
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------------------------------------------
+// The same FP32C Hessian with lanes over halos (launch shape LT_PX_HALO_LANES): a wavefront takes a
+// row segment of SEG pixels and walks each run 64 halos at a time, lane l holding halo c0 + l, so
+// every halo constant of the FP32 terms is a VGPR operand (on gfx950 an FP32 instruction with an
+// SGPR operand issues at half the rate of an all-VGPR one) and the near-core test is a compare and
+// select per (halo, pixel), with no branch. Each lane keeps FP32 sums of its halos' terms for the SEG
+// pixels (at most FLUSH halos per sum, as the chunks of the pixel-per-lane kernel); they are summed
+// over the 64 lanes in FP64 through LDS into the FP64 run sums of lane j = pixel j. The (halo, pixel)
+// pairs inside a halo's near radius are masked out of the FP32 sums and added from the FP64 terms,
+// lane j = pixel j, for the halos whose near disk reaches the segment (the same FP32 near test, so
+// each pair is taken exactly once). Same epilogue and det-A guard as k_hessian32.
+// ------------------------------------------------------------------------------------------------
+constexpr int SEG = 16;
+constexpr int FLUSH = 8;  // chunks of 64 halos between two FP64 flushes: <= 8 FP32 terms per lane sum
+
+// one FP32 term of halo record h at lab-frame offsets (tx, row part rw), as (T, D, B) in the lab frame
+template <int TYPE>
+__device__ __forceinline__ void term32(const HaloRec32& h, const Row32& rw, float tx, float& T, float& D, float& B)
+{
+    T = D = B = 0.f;
+    if constexpr (TYPE == TYPE_SIS_CIRC) {
+        hess_sis32_circ(h, tx, rw.ty, rw.ty2, T, D, B);
+    } else {
+        const Frame32 f = frame32(h, rw, tx);
+        if constexpr (TYPE == TYPE_DPIE) hess_piemd32<true>(h, f, T, D, B);
+        else if constexpr (TYPE == TYPE_PIEMD) hess_piemd32<false>(h, f, T, D, B);
+        else hess_sis32(h, f, T, D, B);
+    }
+}
+
+template <int TYPE>
+__device__ __forceinline__ void term64(const HaloRec& h, double px, double py, double& sT, double& sD, double& sB)
+{
+    if constexpr (TYPE == TYPE_DPIE) hess_piemd<true>(h, px, py, sT, sD, sB);
+    else if constexpr (TYPE == TYPE_PIEMD) hess_piemd<false>(h, px, py, sT, sD, sB);
+    else if constexpr (TYPE == TYPE_SIS_CIRC) hess_sis_circ(h, px, py, sT, sD, sB);
+    else hess_sis(h, px, py, sT, sD, sB);
+}
+
+// sum over the 64 lanes of one component's lane sums a[j] (j < SEG) into pixel lane j's FP64 sum s, in
+// lane order, through this wavefront's LDS slice (layout [source lane][pixel])
+__device__ __forceinline__ void flush_component(float* __restrict__ lds, const float (&a)[SEG], int lane, double& s)
+{
+#pragma unroll
+    for (int j = 0; j < SEG; ++j) lds[lane * SEG + j] = a[j];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane < SEG)
+        for (int l = 0; l < WAVE; ++l) s += (double)lds[l * SEG + lane];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ void flush_lanes(float* __restrict__ lds, float (&aT)[SEG], float (&aD)[SEG],
+                                            float (&aB)[SEG], int lane, double& sT, double& sD, double& sB)
+{
+    flush_component(lds, aT, lane, sT);
+    flush_component(lds, aD, lane, sD);
+    flush_component(lds, aB, lane, sB);
+#pragma unroll
+    for (int j = 0; j < SEG; ++j) aT[j] = aD[j] = aB[j] = 0.f;
+}
+
+template <int TYPE>
+__device__ __forceinline__ void run32h(const HaloRec* __restrict__ h64, const HaloRec32* __restrict__ h32, int start,
+                                       int end, const double (&pxs)[SEG], double pxl, double xa, double xb, double py,
+                                       int lane, float* __restrict__ lds, double& sT, double& sD, double& sB)
+{
+    float aT[SEG], aD[SEG], aB[SEG];
+#pragma unroll
+    for (int j = 0; j < SEG; ++j) aT[j] = aD[j] = aB[j] = 0.f;
+    int pending = 0;
+    for (int c0 = start; c0 < end; c0 += WAVE) {
+        const int i = c0 + lane;
+        const bool live = i < end;
+        HaloRec32 h{};
+        double xh = 0., yh = 0.;
+        if (live) {
+            h = h32[i];
+            xh = h64[i].xh;
+            yh = h64[i].yh;
+        } else {
+            h.near2 = -1.f;  // never near, never a candidate; its amplitudes are zero
+        }
+        const Row32 rw = row32(h, yh, py);
+        // near disk reaching the segment [xa, xb] x {py} (FP64, with a margin over the FP32 test)
+        const double dxs = __builtin_fmax(0., __builtin_fmax(xa - xh, xh - xb));
+        const double dys = py - yh;
+        const bool cand = live && __builtin_fma(dxs, dxs, dys * dys) < (double)h.near2 * (1. + 1e-5) + 1e-30;
+#pragma unroll
+        for (int j = 0; j < SEG; ++j) {
+            const float tx = offset_x(xh, pxs[j]);
+            float T, D, B;
+            term32<TYPE>(h, rw, tx, T, D, B);
+            const bool keep = !(__builtin_fmaf(tx, tx, rw.ty2) < h.near2) && live;
+            T = keep ? T : 0.f;
+            D = keep ? D : 0.f;
+            B = keep ? B : 0.f;
+            acc_lab(h, T, D, B, aT[j], aD[j], aB[j]);
+        }
+        if (++pending == FLUSH) {
+            flush_lanes(lds, aT, aD, aB, lane, sT, sD, sB);
+            pending = 0;
+        }
+        // FP64 terms of the near pairs, candidate halos in order, lane j = pixel j
+        uint64_t mask = __builtin_amdgcn_ballot_w64(cand);
+        while (mask) {
+            const int c = __builtin_ctzll(mask);
+            mask &= mask - 1;
+            const double cxh = __shfl(xh, c);
+            const float cty2 = __shfl(rw.ty2, c);
+            const float cnear2 = __shfl(h.near2, c);
+            const float tx = offset_x(cxh, pxl);
+            if (lane < SEG && __builtin_fmaf(tx, tx, cty2) < cnear2) term64<TYPE>(h64[c0 + c], pxl, py, sT, sD, sB);
+        }
+    }
+    if (pending) flush_lanes(lds, aT, aD, aB, lane, sT, sD, sB);
+}
+
+template <int KIND>
+__global__ __launch_bounds__(WAVE* ROWS) void k_hessian32h(const HaloRec* __restrict__ h64,
+                                                           const HaloRec32* __restrict__ h32,
+                                                           const Run* __restrict__ runs, int nruns, GridDesc g,
+                                                           OutDesc o)
+{
+    __shared__ float lds_all[ROWS][WAVE * SEG];
+    const int row = blockIdx.y * ROWS + threadIdx.y;
+    if (row >= g.ny) return;  // the whole wavefront: no work-group barrier is used
+    float* lds = lds_all[threadIdx.y];
+    h64 = select_sample(g, h64, o);
+    h32 += (int64_t)blockIdx.z * g.halo_stride;
+    const int lane = threadIdx.x;
+    const int col0 = blockIdx.x * SEG;
+    const double py = pixel_coord(g.ymin, row + g.jstart, g.dy);
+    // this lane's pixel (lanes 0 .. SEG-1), and all SEG pixel coordinates (wavefront-uniform)
+    const double pxl = pixel_coord(g.xmin, col0 + (lane < SEG ? lane : 0) + g.istart, g.dx);
+    double pxs[SEG];
+#pragma unroll
+    for (int j = 0; j < SEG; ++j) pxs[j] = pixel_coord(g.xmin, col0 + j + g.istart, g.dx);
+    const double xa = pxs[0], xb = pxs[SEG - 1];
+    double tT = 0., tD = 0., tB = 0.;
+    for (int r = 0; r < nruns; ++r) {
+        const Run run = runs[r];
+        double sT = 0., sD = 0., sB = 0.;
+        const int end = run.start + run.count;
+        if (run.type == TYPE_SHEAR) {
+            for (int i = run.start; i < end; ++i) {
+                const HaloRec& h = h64[i];
+                sT += h.h14a + h.h14c;
+                sD += h.h14a - h.h14c;
+                sB += h.h14b;
+            }
+        } else if (run.type == TYPE_DPIE) {
+            run32h<TYPE_DPIE>(h64, h32, run.start, end, pxs, pxl, xa, xb, py, lane, lds, sT, sD, sB);
+        } else if (run.type == TYPE_PIEMD) {
+            run32h<TYPE_PIEMD>(h64, h32, run.start, end, pxs, pxl, xa, xb, py, lane, lds, sT, sD, sB);
+        } else if (run.circ) {
+            run32h<TYPE_SIS_CIRC>(h64, h32, run.start, end, pxs, pxl, xa, xb, py, lane, lds, sT, sD, sB);
+        } else {
+            run32h<TYPE_SIS>(h64, h32, run.start, end, pxs, pxl, xa, xb, py, lane, lds, sT, sD, sB);
+        }
+        tT = __builtin_fma(run.weight, sT, tT);
+        tD = __builtin_fma(run.weight, sD, tD);
+        tB = __builtin_fma(run.weight, sB, tB);
+    }
+    const int col = col0 + lane;
+    if (lane < SEG && col < g.nx) {
+        if constexpr (KIND == LT_MAP_FIELDS || KIND == LT_MAP_AMPLI || KIND == LT_MAP_SHEAR) {
+            const double kap = 0.5 * tT * o.r;
+            const double gb = 0.5 * (__builtin_fabs(tD) + 2. * __builtin_fabs(tB)) * __builtin_fabs(o.r);
+            if (nruns > 0 && __builtin_fabs(1. - kap) + gb > kDetGuard) hessian_fp64_pixel(h64, runs, nruns, pxl, py, tT, tD, tB);
+        }
+        hess_epilogue<KIND>(o, (int64_t)row * g.pitch + col, tT, tD, tB);
+    }
+}
+
+static hipError_t launch_hessian32h(const HaloRec* h64, const HaloRec32* h32, const Run* runs, int nruns,
+                                    const GridDesc& g, const OutDesc& o, hipStream_t s)
+{
+    dim3 blk(WAVE, ROWS, 1);
+    dim3 grd((unsigned)((g.nx + SEG - 1) / SEG), (unsigned)((g.ny + ROWS - 1) / ROWS),
+             (unsigned)(g.nsamples > 0 ? g.nsamples : 1));
+    switch (o.kind) {
+    case LT_MAP_FIELDS: hipLaunchKernelGGL((k_hessian32h<LT_MAP_FIELDS>), grd, blk, 0, s, h64, h32, runs, nruns, g, o); break;
+    case LT_MAP_AMPLI: hipLaunchKernelGGL((k_hessian32h<LT_MAP_AMPLI>), grd, blk, 0, s, h64, h32, runs, nruns, g, o); break;
+    case LT_MAP_SHEAR: hipLaunchKernelGGL((k_hessian32h<LT_MAP_SHEAR>), grd, blk, 0, s, h64, h32, runs, nruns, g, o); break;
+    case LT_MAP_MASS: hipLaunchKernelGGL((k_hessian32h<LT_MAP_MASS>), grd, blk, 0, s, h64, h32, runs, nruns, g, o); break;
+    default: hipLaunchKernelGGL((k_hessian32h<LT_MAP_HESSIAN>), grd, blk, 0, s, h64, h32, runs, nruns, g, o); break;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_hessian32(const HaloRec* h64, const HaloRec32* h32, const Run* runs, int nruns, const GridDesc& g,
                             const OutDesc& o, int px, hipStream_t s)
 {
     switch (px) {
     case 1: return launch_hessian32_px<1>(h64, h32, runs, nruns, g, o, s);
     case 4: return launch_hessian32_px<4>(h64, h32, runs, nruns, g, o, s);
+    case LT_PX_HALO_LANES: return launch_hessian32h(h64, h32, runs, nruns, g, o, s);
     default: return launch_hessian32_px<2>(h64, h32, runs, nruns, g, o, s);
     }
 }
@@ -576,6 +772,142 @@ hipError_t launch_gradient32(const HaloRec* h64, const HaloRec32* recs32, const 
     case 1: hipLaunchKernelGGL(k_gradient32<1>, grid_for(g, 1), blk, 0, s, h64, g32, runs, nruns, g, o); break;
     case 4: hipLaunchKernelGGL(k_gradient32<4>, grid_for(g, 4), blk, 0, s, h64, g32, runs, nruns, g, o); break;
     default: hipLaunchKernelGGL(k_gradient32<2>, grid_for(g, 2), blk, 0, s, h64, g32, runs, nruns, g, o); break;
+    }
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------------
+// FP32-compensated potential: the algebraic I_{0.5} potential of lt_terms64.h (pot_halo: two
+// logarithms and two arguments per core radius instead of the reference's complex cosh / sinh / log,
+// src/potential_GPU.cu:47-116) evaluated in FP32 with the single-precision arctangent and the
+// hardware base-2 logarithm; the offset pixel - centre formed in FP64 and rounded once; FP32 terms
+// summed per chunk of CHUNK halos into FP64 run sums. Every term is FP32 (the potential has no
+// singular core: it is finite and smooth at the centre of a cored halo), and the map reports its
+// error against the FP64 kernel.
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ void log_ratio32(float ur, float ui, float vr, float vi, float& lr, float& li)
+{
+    lr = 0.5f * logf_lt(__builtin_fmaf(ur, ur, ui * ui) * rcp32(__builtin_fmaf(vr, vr, vi * vi)));
+    li = atan2f_lt(__builtin_fmaf(ui, vr, -(ur * vi)), __builtin_fmaf(ur, vr, ui * vi));
+}
+
+struct PotShared32 {
+    float p, q, PC, pq, Nr, Ni, rem2, isrem, srem;
+};
+
+__device__ __forceinline__ float pi05_core32(const PotShared32& s, float x, float y, float rc, float irc, float rc2)
+{
+    const float W2 = rc2 + s.rem2;
+    const float W = W2 * rsq32(W2);
+    const float onePC = 1.f + s.PC;
+    float a1r, a1i, a2r, a2i;
+    log_ratio32(onePC, -s.pq, __builtin_fmaf(W, irc, s.PC), -s.pq, a1r, a1i);
+    const float t = s.srem * rcp32(rc + W);
+    log_ratio32(__builtin_fmaf(t, s.Nr, onePC), __builtin_fmaf(t, s.Ni, -s.pq), __builtin_fmaf(-t, s.Nr, onePC),
+                __builtin_fmaf(-t, s.Ni, -s.pq), a2r, a2i);
+    const float sz = s.srem * irc;
+    const float cr = __builtin_fmaf(s.Nr, a1r, -(s.Ni * a1i)) + sz * a2r;
+    const float ci = __builtin_fmaf(s.Nr, a1i, s.Ni * a1r) + sz * a2i;
+    return rc * s.isrem * __builtin_fmaf(ci, x, -(cr * y));
+}
+
+template <int TYPE>
+__device__ __forceinline__ float pot32(const PotRec32& h, float tx, float ty)
+{
+    const float x = __builtin_fmaf(tx, h.cs, ty * h.sn);
+    const float y = __builtin_fmaf(ty, h.cs, -(tx * h.sn));
+    const float x2 = x * x, y2 = y * y;
+    if constexpr (TYPE == TYPE_SIS) {
+        const float R2 = __builtin_fmaf(h.sa, x2, h.sb * y2);
+        return h.amp * (R2 * rsq32(R2));
+    } else {
+        PotShared32 s;
+        const float iz = rsq32(x2 + y2);
+        s.p = h.pkp * y * iz;
+        s.q = h.pkq * x * iz;
+        const float P2 = __builtin_fmaf(s.p, s.p, 1.f), C2 = __builtin_fmaf(-s.q, s.q, 1.f);
+        const float P = P2 * rsq32(P2), C = C2 * rsq32(C2);
+        s.PC = P * C;
+        s.pq = s.p * s.q;
+        s.Nr = -s.p * C;
+        s.Ni = P * s.q;
+        s.rem2 = __builtin_fmaf(x2, h.kx, y2 * h.ky);
+        s.isrem = rsq32(s.rem2);
+        s.srem = s.rem2 * s.isrem;
+        const float pa = pi05_core32(s, x, y, h.ra, h.ira, h.ra2);
+        if constexpr (TYPE == TYPE_PIEMD) return h.amp * pa;
+        const float ps = pi05_core32(s, x, y, h.rb, h.irb, h.rb2);
+        return h.amp * (pa - ps);
+    }
+}
+
+template <int TYPE, int PX>
+__device__ __forceinline__ void runp32(const HaloRec* __restrict__ h64, const PotRec32* __restrict__ p32, int start,
+                                       int end, const double* px, double py, double* s)
+{
+    for (int c0 = start; c0 < end; c0 += CHUNK) {
+        const int c1 = min(c0 + CHUNK, end);
+        float f[PX];
+#pragma unroll
+        for (int k = 0; k < PX; ++k) f[k] = 0.f;
+        for (int i = c0; i < c1; ++i) {
+            const PotRec32& h = p32[i];
+            const double xh = h64[i].xh;
+            const float ty = (float)(py - h64[i].yh);
+#pragma unroll
+            for (int k = 0; k < PX; ++k) f[k] += pot32<TYPE>(h, offset_x(xh, px[k]), ty);
+        }
+#pragma unroll
+        for (int k = 0; k < PX; ++k) s[k] += (double)f[k];
+    }
+}
+
+template <int PX>
+__global__ __launch_bounds__(WAVE* ROWS) void k_potential32(const HaloRec* __restrict__ h64,
+                                                            const PotRec32* __restrict__ p32,
+                                                            const Run* __restrict__ runs, int nruns, GridDesc g,
+                                                            OutDesc o)
+{
+    const int row = blockIdx.y * ROWS + threadIdx.y;
+    if (row >= g.ny) return;
+    h64 = select_sample(g, h64, o);
+    p32 += (int64_t)blockIdx.z * g.halo_stride;
+    const int col0 = blockIdx.x * (WAVE * PX) + threadIdx.x;
+    const double py = pixel_coord(g.ymin, row + g.jstart, g.dy);
+    double px[PX], tot[PX];
+#pragma unroll
+    for (int k = 0; k < PX; ++k) {
+        px[k] = pixel_coord(g.xmin, col0 + k * WAVE + g.istart, g.dx);
+        tot[k] = 0.;
+    }
+    for (int r = 0; r < nruns; ++r) {
+        const Run run = runs[r];
+        double s[PX];
+#pragma unroll
+        for (int k = 0; k < PX; ++k) s[k] = 0.;
+        const int end = run.start + run.count;
+        if (run.type == TYPE_DPIE) runp32<TYPE_DPIE, PX>(h64, p32, run.start, end, px, py, s);
+        else if (run.type == TYPE_PIEMD) runp32<TYPE_PIEMD, PX>(h64, p32, run.start, end, px, py, s);
+        else runp32<TYPE_SIS, PX>(h64, p32, run.start, end, px, py, s);
+#pragma unroll
+        for (int k = 0; k < PX; ++k) tot[k] += s[k];
+    }
+#pragma unroll
+    for (int k = 0; k < PX; ++k) {
+        const int col = col0 + k * WAVE;
+        if (col < g.nx) o.p0[(int64_t)row * g.pitch + col] = tot[k] * o.mult;
+    }
+}
+
+hipError_t launch_potential32(const HaloRec* h64, const HaloRec32* recs32, const Run* runs, int nruns,
+                              const GridDesc& g, const OutDesc& o, int px, hipStream_t s)
+{
+    const PotRec32* p32 = reinterpret_cast<const PotRec32*>(recs32);
+    dim3 blk(WAVE, ROWS, 1);
+    switch (px) {
+    case 1: hipLaunchKernelGGL(k_potential32<1>, grid_for(g, 1), blk, 0, s, h64, p32, runs, nruns, g, o); break;
+    case 4: hipLaunchKernelGGL(k_potential32<4>, grid_for(g, 4), blk, 0, s, h64, p32, runs, nruns, g, o); break;
+    default: hipLaunchKernelGGL(k_potential32<2>, grid_for(g, 2), blk, 0, s, h64, p32, runs, nruns, g, o); break;
     }
     return hipGetLastError();
 }

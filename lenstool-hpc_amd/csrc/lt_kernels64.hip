@@ -166,24 +166,40 @@ __global__ __launch_bounds__(WAVE* ROWS) void k_gradient64(const HaloRec* __rest
     }
 }
 
+template <int PX>
 __global__ __launch_bounds__(WAVE* ROWS) void k_potential64(const HaloRec* __restrict__ halos,
                                                             const Run* __restrict__ runs, int nruns,
                                                             GridDesc g, OutDesc o)
 {
     const int row = blockIdx.y * ROWS + threadIdx.y;
-    const int col = blockIdx.x * WAVE + threadIdx.x;
-    if (row >= g.ny || col >= g.nx) return;
+    if (row >= g.ny) return;
     halos = select_sample(g, halos, o);
-    const double px = pixel_coord(g.xmin, col + g.istart, g.dx);
+    const int col0 = blockIdx.x * (WAVE * PX) + threadIdx.x;
     const double py = pixel_coord(g.ymin, row + g.jstart, g.dy);
-    double tot = 0.;
+    double px[PX], tot[PX];
+#pragma unroll
+    for (int k = 0; k < PX; ++k) {
+        px[k] = pixel_coord(g.xmin, col0 + k * WAVE + g.istart, g.dx);
+        tot[k] = 0.;
+    }
     for (int r = 0; r < nruns; ++r) {
         const Run run = runs[r];
-        double s = 0.;
-        for (int i = run.start; i < run.start + run.count; ++i) s += pot_halo(halos[i], run.type, px, py);
-        tot += s;
+        double s[PX];
+#pragma unroll
+        for (int k = 0; k < PX; ++k) s[k] = 0.;
+        for (int i = run.start; i < run.start + run.count; ++i) {
+            const HaloRec& h = halos[i];
+#pragma unroll
+            for (int k = 0; k < PX; ++k) s[k] += pot_halo(h, run.type, px[k], py);
+        }
+#pragma unroll
+        for (int k = 0; k < PX; ++k) tot[k] += s[k];
     }
-    o.p0[(int64_t)row * g.pitch + col] = tot * o.mult;
+#pragma unroll
+    for (int k = 0; k < PX; ++k) {
+        const int col = col0 + k * WAVE;
+        if (col < g.nx) o.p0[(int64_t)row * g.pitch + col] = tot[k] * o.mult;
+    }
 }
 
 // Gradient at arbitrary points for a batch of lens models (blockIdx.y = sample), e.g. at the
@@ -305,10 +321,14 @@ hipError_t launch_gradient64(const HaloRec* halos, const Run* runs, int nruns, c
 }
 
 hipError_t launch_potential64(const HaloRec* halos, const Run* runs, int nruns, const GridDesc& g,
-                              const OutDesc& o, hipStream_t s)
+                              const OutDesc& o, int px, hipStream_t s)
 {
     dim3 blk(WAVE, ROWS, 1);
-    hipLaunchKernelGGL(k_potential64, grid_for(g, 1), blk, 0, s, halos, runs, nruns, g, o);
+    switch (px) {
+    case 1: hipLaunchKernelGGL(k_potential64<1>, grid_for(g, 1), blk, 0, s, halos, runs, nruns, g, o); break;
+    case 4: hipLaunchKernelGGL(k_potential64<4>, grid_for(g, 4), blk, 0, s, halos, runs, nruns, g, o); break;
+    default: hipLaunchKernelGGL(k_potential64<2>, grid_for(g, 2), blk, 0, s, halos, runs, nruns, g, o); break;
+    }
     return hipGetLastError();
 }
 

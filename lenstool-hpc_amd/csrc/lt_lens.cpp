@@ -217,6 +217,29 @@ int pack_model(const lt_potential_soa* L, int n, Use use, std::vector<HaloRec>& 
                 g.circ = (float)h.circ;
             }
             std::memcpy(&f, &g, sizeof(g));
+        } else if (use == Use::Potential) {
+            // potential models carry the potential's FP32 constants
+            PotRec32 q{};
+            q.cs = (float)cs;
+            q.sn = (float)sn;
+            if (piemd) {
+                q.kx = (float)h.kx;
+                q.ky = (float)h.ky;
+                q.pkp = (float)h.pkp;
+                q.pkq = (float)h.pkq;
+                q.ra = (float)h.rc;
+                q.ira = (float)h.irc;
+                q.ra2 = (float)h.ra2;
+                q.rb = (float)h.rcut;
+                q.irb = (float)h.ircut;
+                q.rb2 = (float)h.rb2;
+                q.amp = (float)h.ampg;
+            } else if (t == TYPE_SIS) {
+                q.sa = (float)h.sa;
+                q.sb = (float)h.sb;
+                q.amp = (float)h.b0;
+            }
+            std::memcpy(&f, &q, sizeof(q));
         }
     }
     // runs: equal type (and, for the Kmap, equal dlsds); the scan never reads past n - 1

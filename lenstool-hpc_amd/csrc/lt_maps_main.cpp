@@ -195,7 +195,7 @@ int main(int argc, char** argv)
         const double dx = (frame.xmax - frame.xmin) / (nc - 1), dy = (frame.ymax - frame.ymin) / (nc - 1);
         const int use = j.kind == LT_MAP_DPL ? LT_MAP_GRADIENT : j.kind == LT_MAP_POT ? LT_MAP_POT
                         : j.kind == LT_MAP_MASS ? LT_MAP_MASS : LT_MAP_HESSIAN;
-        const int prec = j.kind == LT_MAP_POT || j.kind == LT_MAP_DPL ? LT_PREC_FP64 : o.precision;
+        const int prec = o.precision;
         const int nplanes = j.kind == LT_MAP_DPL ? 2 : 1;
         const long long npix = (long long)nc * nc;
         // samples per launch: --batch, and at most 4 GiB of map planes (device and host copy)
@@ -290,9 +290,15 @@ int main(int argc, char** argv)
                                          crit.data(), caus.data(), kinds.data(), room, &nseg);
                     if (st == LT_ERR_OVERFLOW) room = nseg;
                 }
-                if (fail(st, "lt_critic_lines")) return 1;
+                // LT_ERR_TYPE: critical lines done, caustics unavailable (external-shear halos)
+                const bool no_caustics = st == LT_ERR_TYPE;
+                if (no_caustics) {
+                    if (!o.quiet) std::fprintf(stderr, "lenstool_maps: %s\n", lt_last_error());
+                } else if (fail(st, "lt_critic_lines")) {
+                    return 1;
+                }
                 const std::string tag = std::to_string(p) + (bayes ? "_" + std::to_string(k) : std::string());
-                for (int w = 0; w < 2; ++w) {
+                for (int w = 0; w < (no_caustics ? 1 : 2); ++w) {
                     const std::string path = o.out + (w ? "/caustic_" : "/critic_") + tag + ".dat";
                     std::FILE* f = std::fopen(path.c_str(), "w");
                     if (!f) {

@@ -185,27 +185,4 @@ __device__ __forceinline__ double pixel_coord(double xmin, int i, double d)
     return xmin + t;
 }
 
-// Error-free transformations for compensated FP32 sums (Knuth two-sum, Dekker fast-two-sum).
-__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e)
-{
-#pragma clang fp contract(off)
-    s = a + b;
-    float bb = s - a;
-    e = (a - (s - bb)) + (b - bb);
-}
-
-// Compensated accumulator: value = hi + lo, with the rounding error of every addition kept.
-struct CompF32 {
-    float hi, lo;
-    __device__ __forceinline__ void init() { hi = 0.f; lo = 0.f; }
-    __device__ __forceinline__ void add(float v)
-    {
-        float s, e;
-        two_sum(hi, v, s, e);
-        hi = s;
-        lo += e;
-    }
-    __device__ __forceinline__ double value() const { return (double)hi + (double)lo; }
-};
-
 }  // namespace lt

@@ -245,6 +245,19 @@ static F select_mode(const F (&table)[N], int mode, const char* what)
     return nullptr;
 }
 
+// select_map_dpl_function, src/grid_map_dpl_GPU.cuh:39 (its body is compiled out in the reference,
+// src/grid_map_dpl_GPU.cu:58-71, which names dpl_1_grid_CPU_GPU for dpl mode 1): mode 1 gives the
+// displacement scaling D_ls/D_s applied in place to `map` (map_grid_dpl_GPU itself ignores its map
+// function and applies that scaling to both planes, src/grid_map_dpl_GPU.cu:75-190).
+void dpl_1_grid_CPU_GPU(type_t* map, type_t* a, type_t* b, type_t* c, type_t* d, type_t dl0s, type_t ds, type_t dl,
+                        type_t h, type_t z, int nx, int ny, const grid_param* frame)
+{
+    (void)a, (void)b, (void)c, (void)d, (void)dl, (void)h, (void)z, (void)frame;
+    report(lt_scale_device(map, dl0s / ds, (long long)nx * ny, nullptr), "dpl_1_grid_CPU_GPU");
+}
+
+static const map_gpu_function_t kDpl[1] = {dpl_1_grid_CPU_GPU};
+map_gpu_function_t select_map_dpl_function(const runmode_param* rm) { return select_mode(kDpl, rm->dpl, "dpl"); }
 map_gpu_function_t select_map_ampli_function(const runmode_param* rm) { return select_mode(kAmpli, rm->amplif, "ampli"); }
 map_gpu_function_t select_map_shear_function(const runmode_param* rm) { return select_mode(kShear, rm->shear, "shear"); }
 map_gpu_function_t select_map_mass_function(const runmode_param* rm) { return select_mode(kMass, rm->mass, "mass"); }
@@ -400,6 +413,127 @@ void map_grid_dpl_GPU(map_gpu_function_t f, type_t* dplx, type_t* dply, const co
     report(lt_map_grid(LT_MAP_DPL, 1, dplx, dply, Cz(cosmo), G(frame), L(lens), nhalos, zl_of(lens), z, dx, dy, nx, ny,
                        istart, jstart, LT_PREC_FP64),
            "map_grid_dpl_GPU");
+}
+
+// ---- the reference's launchers on device data ------------------------------------------------------
+// module_potentialDerivatives_totalGradient_SOA_CPU_GPU (src/grid_gradient_GPU.cu:365-420),
+// module_potentialDerivatives_totalGradient2_SOA_CPU_GPU (src/grid_gradient2_GPU.cu:186-209) and
+// module_potentialDerivatives_Kmap_SOA_CPU_GPU (src/grid_gradient2_GPU.cu:212-236) take device output
+// planes, a grid_param in device memory and a Potential_SOA that itself lives in device memory, whose
+// arrays are device arrays; only type, position_x/y, b0, ellipticity_angle, ellipticity_potential,
+// rcore, rcut, anglecos/sin (and dlsds for the Kmap) are valid there (src/grid_gradient2_GPU.cu:101-147).
+// They read the descriptor and those arrays back, pack the model on the host and run the map kernel
+// into the caller's planes (pitch nx), synchronously as the reference does.
+namespace {
+
+struct HostLens {
+    Potential_SOA soa{};
+    std::vector<int> type;
+    std::vector<double> v[11];
+};
+
+int read_device_array(std::vector<double>& dst, const double* src, int n)
+{
+    dst.assign((size_t)n, 0.0);
+    if (!src || n == 0) return LT_OK;
+    return lt_memcpy_d2h(dst.data(), src, (long long)n * (long long)sizeof(double), nullptr);
+}
+
+int read_device_lens(const Potential_SOA* lens_gpu, int nhalos, bool kmap, HostLens& h)
+{
+    if (!lens_gpu || nhalos < 0) return LT_ERR_INVALID;
+    Potential_SOA d{};
+    int st = lt_memcpy_d2h(&d, lens_gpu, (long long)sizeof(Potential_SOA), nullptr);
+    if (st) return st;
+    if (!d.type || !d.position_x || !d.position_y || !d.b0 || !d.ellipticity_potential || !d.anglecos || !d.anglesin)
+        return LT_ERR_INVALID;
+    h.type.assign((size_t)nhalos, 0);
+    if (nhalos && (st = lt_memcpy_d2h(h.type.data(), d.type, (long long)nhalos * 4, nullptr))) return st;
+    const double* src[11] = {d.position_x, d.position_y,      d.b0,       d.ellipticity_angle,
+                             d.ellipticity_potential, d.rcore, d.rcut, d.anglecos,
+                             d.anglesin,   kmap ? d.dlsds : nullptr, nullptr};
+    for (int k = 0; k < 10; ++k)
+        if ((st = read_device_array(h.v[k], src[k], nhalos))) return st;
+    h.soa.type = h.type.data();
+    h.soa.position_x = h.v[0].data();
+    h.soa.position_y = h.v[1].data();
+    h.soa.b0 = h.v[2].data();
+    h.soa.ellipticity_angle = d.ellipticity_angle ? h.v[3].data() : nullptr;
+    h.soa.ellipticity_potential = h.v[4].data();
+    h.soa.rcore = d.rcore ? h.v[5].data() : nullptr;
+    h.soa.rcut = d.rcut ? h.v[6].data() : nullptr;
+    h.soa.anglecos = h.v[7].data();
+    h.soa.anglesin = h.v[8].data();
+    h.soa.dlsds = (kmap && d.dlsds) ? h.v[9].data() : nullptr;
+    return LT_OK;
+}
+
+// kind LT_MAP_GRADIENT (p0, p1) or LT_MAP_HESSIAN (p0..p3); use LT_MAP_GRADIENT, LT_MAP_HESSIAN or
+// LT_MAP_MASS (Kmap runs)
+void device_launch(int kind, int use, type_t* p0, type_t* p1, type_t* p2, type_t* p3, const grid_param* frame_gpu,
+                   const Potential_SOA* lens_gpu, int nhalos, type_t dx, type_t dy, int nx, int ny, int istart,
+                   int jstart, bool square, const char* fn)
+{
+    HostLens h;
+    grid_param frame{};
+    int st = frame_gpu ? lt_memcpy_d2h(&frame, frame_gpu, (long long)sizeof(grid_param), nullptr) : LT_ERR_INVALID;
+    if (st == LT_OK && square) {
+        if (nx < 2) st = LT_ERR_INVALID;
+        dx = (frame.xmax - frame.xmin) / (nx - 1);
+        dy = (frame.ymax - frame.ymin) / (ny - 1);
+    }
+    if (st == LT_OK) st = read_device_lens(lens_gpu, nhalos, use == LT_MAP_MASS, h);
+    lt_model m = nullptr;
+    if (st == LT_OK) st = lt_model_create(&m, L(&h.soa), nhalos, use);
+    if (st == LT_OK) {
+        lt_grid_desc g{frame.xmin, frame.ymin, dx, dy, nx, ny, istart, jstart, nx};
+        st = lt_map_device(m, kind, 0, p0, p1, p2, p3, nullptr, 1.0, 1.0, &g, LT_PREC_FP64, 0, nullptr);
+    }
+    if (st == LT_OK) st = lt_stream_sync(nullptr);
+    if (m) lt_model_destroy(m);
+    report(st, fn);
+}
+
+}  // namespace
+
+void module_potentialDerivatives_totalGradient_SOA_CPU_GPU(type_t* gx, type_t* gy, const grid_param* frame,
+                                                           const Potential_SOA* lens_gpu, int nhalos, type_t dx,
+                                                           type_t dy, int nx, int ny, int istart, int jstart)
+{
+    device_launch(LT_MAP_GRADIENT, LT_MAP_GRADIENT, gx, gy, nullptr, nullptr, frame, lens_gpu, nhalos, dx, dy, nx, ny,
+                  istart, jstart, false, "module_potentialDerivatives_totalGradient_SOA_CPU_GPU");
+}
+
+void module_potentialDerivatives_totalGradient_SOA_CPU_GPU(type_t* gx, type_t* gy, const grid_param* frame,
+                                                           const Potential_SOA* lens_gpu, int nbgridcells, int nhalos)
+{
+    device_launch(LT_MAP_GRADIENT, LT_MAP_GRADIENT, gx, gy, nullptr, nullptr, frame, lens_gpu, nhalos, 0, 0,
+                  nbgridcells, nbgridcells, 0, 0, true, "module_potentialDerivatives_totalGradient_SOA_CPU_GPU");
+}
+
+void module_potentialDerivatives_totalGradient2_SOA_CPU_GPU(type_t* a, type_t* b, type_t* c, type_t* d,
+                                                            const grid_param* frame, const Potential_SOA* lens_gpu,
+                                                            int nhalos, type_t dx, type_t dy, int nx, int ny,
+                                                            int istart, int jstart)
+{
+    device_launch(LT_MAP_HESSIAN, LT_MAP_HESSIAN, a, b, c, d, frame, lens_gpu, nhalos, dx, dy, nx, ny, istart, jstart,
+                  false, "module_potentialDerivatives_totalGradient2_SOA_CPU_GPU");
+}
+
+void module_potentialDerivatives_totalGradient2_SOA_CPU_GPU(type_t* a, type_t* b, type_t* c, type_t* d,
+                                                            const grid_param* frame, const Potential_SOA* lens_gpu,
+                                                            int nbgridcells, int nhalos)
+{
+    device_launch(LT_MAP_HESSIAN, LT_MAP_HESSIAN, a, b, c, d, frame, lens_gpu, nhalos, 0, 0, nbgridcells, nbgridcells,
+                  0, 0, true, "module_potentialDerivatives_totalGradient2_SOA_CPU_GPU");
+}
+
+void module_potentialDerivatives_Kmap_SOA_CPU_GPU(type_t* a, type_t* b, type_t* c, type_t* d, const grid_param* frame,
+                                                  const Potential_SOA* lens_gpu, int nhalos, type_t dx, type_t dy,
+                                                  int nx, int ny, int istart, int jstart)
+{
+    device_launch(LT_MAP_HESSIAN, LT_MAP_MASS, a, b, c, d, frame, lens_gpu, nhalos, dx, dy, nx, ny, istart, jstart,
+                  false, "module_potentialDerivatives_Kmap_SOA_CPU_GPU");
 }
 
 // ---- delensing and chi^2 -----------------------------------------------------------------------------

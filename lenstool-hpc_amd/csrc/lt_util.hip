@@ -1,6 +1,8 @@
 // lt_util.hip -- small utility kernels.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "lt_internal.h"
 #include "lt_kernels.h"
 
@@ -63,6 +65,34 @@ hipError_t launch_fill_i32(int32_t* p, int32_t v, size_t n, hipStream_t s)
 {
     if (!n) return hipSuccess;
     hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(n)), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+// Byte fill: 16-byte vectors over the aligned body, single bytes at both ends.
+__global__ __launch_bounds__(256) void k_fill16(uint4* __restrict__ p, uint4 v, size_t nvec)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nvec; i += stride) p[i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_fill1(unsigned char* __restrict__ p, unsigned char v, size_t n)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+hipError_t launch_fill_bytes(void* dst, unsigned char v, size_t bytes, hipStream_t s)
+{
+    unsigned char* p = (unsigned char*)dst;
+    const size_t head = std::min(bytes, (size_t)((16 - (uintptr_t)p % 16) % 16));
+    const size_t nvec = (bytes - head) / 16;
+    const size_t tail = bytes - head - nvec * 16;
+    const uint32_t w = 0x01010101u * v;
+    if (head) hipLaunchKernelGGL(k_fill1, dim3(1), dim3(256), 0, s, p, v, head);
+    if (nvec)
+        hipLaunchKernelGGL(k_fill16, dim3(blocks_for(nvec)), dim3(256), 0, s, (uint4*)(p + head), make_uint4(w, w, w, w),
+                           nvec);
+    if (tail) hipLaunchKernelGGL(k_fill1, dim3(1), dim3(256), 0, s, p + head + nvec * 16, v, tail);
     return hipGetLastError();
 }
 

@@ -16,9 +16,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblenstool_hip.so")
 
 LT_OK = 0
+LT_ERR_TYPE = 2
 LT_ERR_OVERFLOW = 5
 PREC_FP64 = 0
 PREC_FP32C = 1
+PX_HALO_LANES = 16
 MAP_AMPLI, MAP_SHEAR, MAP_MASS, MAP_POT, MAP_DPL, MAP_FIELDS, MAP_HESSIAN, MAP_GRADIENT = range(1, 9)
 
 _lib = None
@@ -594,7 +596,9 @@ def critic_device(det_ptr, kappa_ptr, grid: M.Grid, max_segments, stream=0, ista
 
 def critic_lines(lens: M.LensModel, frame, nx, ny, cosmo, z_lens, z_source, precision=PREC_FP64,
                  max_segments=1 << 20, caustics=True):
-    """Critical lines (n, 4), caustics (n, 4) or None, kinds (n,) of a lens; total segment count."""
+    """Critical lines (n, 4), caustics (n, 4) or None, kinds (n,) of a lens; total segment count.
+    A model with external-shear (type 14) halos gets its lines and NaN caustics (status LT_ERR_TYPE
+    from the library, not raised here)."""
     crit = np.zeros((max_segments, 4))
     caus = np.zeros((max_segments, 4)) if caustics else None
     kinds = np.zeros(max_segments, dtype=np.int32)
@@ -602,7 +606,7 @@ def critic_lines(lens: M.LensModel, frame, nx, ny, cosmo, z_lens, z_source, prec
     st = lib().lt_critic_lines(C.byref(cosmo), C.byref(frame), C.byref(lens.soa()), lens.n, z_lens, z_source, nx, ny,
                                precision, M.dptr(crit), M.dptr(caus) if caus is not None else None,
                                kinds.ctypes.data_as(C.POINTER(C.c_int)), max_segments, C.byref(total))
-    if st not in (LT_OK, LT_ERR_OVERFLOW):
+    if st not in (LT_OK, LT_ERR_OVERFLOW, LT_ERR_TYPE):
         check(st)
     got = min(total.value, max_segments)
     return crit[:got], (caus[:got] if caus is not None else None), kinds[:got], total.value

@@ -82,11 +82,34 @@ def lib():
     return _lib
 
 
-def ref_lib():
-    """The reference's own CPU functions (oracle/_ref), or None when they were not built."""
-    global _ref
-    if _ref is None:
-        path = os.path.join(_HERE, "_ref", "libltref.so")
+REF_FLAGS = {"libltref.so": "g++ 11 -O3 -march=x86-64-v3 -fopenmp -std=c++14 -D_double",
+             "libltref_v4.so": "g++ 11 -O3 -march=x86-64-v4 -U__AVX512F__ -fopenmp -std=c++14 -D_double"}
+
+
+def host_has_avx512():
+    try:
+        flags = next(ln for ln in open("/proc/cpuinfo") if ln.startswith("flags")).split()
+    except (OSError, StopIteration):
+        return False
+    return all(f in flags for f in ("avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"))
+
+
+def ref_builds():
+    """File names of the reference builds in oracle/_ref that this host can run."""
+    out = []
+    for name in ("libltref.so", "libltref_v4.so"):
+        if os.path.exists(os.path.join(_HERE, "_ref", name)) and (name == "libltref.so" or host_has_avx512()):
+            out.append(name)
+    return out
+
+
+_refs = {}
+
+
+def ref_lib(name="libltref.so"):
+    """The reference's own CPU functions (oracle/_ref/<name>), or None when they were not built."""
+    if name not in _refs:
+        path = os.path.join(_HERE, "_ref", name)
         if not os.path.exists(path):
             return None
         L = C.CDLL(path)
@@ -100,8 +123,8 @@ def ref_lib():
         L.ref_chi_computation.argtypes = [dp, P(C.c_int), P(C.c_int), P(M.LtPoint), P(C.c_int),
                                           P(M.LtGalaxy), C.c_int]
         L.ref_set_threads.argtypes = [C.c_int]
-        _ref = L
-    return _ref
+        _refs[name] = L
+    return _refs[name]
 
 
 def _chk(st, what):

@@ -8,7 +8,11 @@
 //         potential mode 1, dplx, dply; the CPU-named entry points: gradient2_grid_CPU (AoS, dx/dy
 //         overload, istart=1, jstart=2) a and c, amplif_grid_CPU mode 5, gradient_grid_CPU (square) x;
 //         a plane whose first 16 values are module_potentialDerivatives_totalGradient2_SOA (a,b,c,d)
-//         at the first 4 pixels of the diagonal; then the int status of lenstool_gpu_last_status().
+//         at the first 4 pixels of the diagonal; the launchers on device data (device planes, frame and
+//         Potential_SOA descriptor): totalGradient x, y (dx/dy overload, istart=1, jstart=2),
+//         totalGradient2 a, b, c, d (same), Kmap a, c, totalGradient x (square overload),
+//         totalGradient2 a (square overload), dpl mode 1 from select_map_dpl_function applied to
+//         the square gradient's x plane; then the int status of lenstool_gpu_last_status().
 #include <cmath>
 #include <cstdio>
 #include <algorithm>
@@ -16,6 +20,17 @@
 #include <vector>
 
 #include "lenstool_hip/lenstool_gpu.hpp"
+#include "lenstool_hip/lt_capi.h"
+
+// a host array copied to a new device array
+template <class T>
+static T* to_device(const std::vector<T>& v)
+{
+    T* d = nullptr;
+    if (lt_dev_alloc((void**)&d, (long long)(v.size() * sizeof(T)))) std::exit(6);
+    if (lt_memcpy_h2d(d, v.data(), (long long)(v.size() * sizeof(T)), nullptr)) std::exit(6);
+    return d;
+}
 
 static void foreign_ampli1(type_t* map, type_t* a, type_t* b, type_t* c, type_t* d, type_t dl0s, type_t dos, type_t dl,
                            type_t h, type_t z, int nx, int ny, const grid_param* frame)
@@ -130,6 +145,61 @@ int main(int argc, char** argv)
         p0[4 * k + 3] = m.d;
     }
     put(p0);
+
+    // the reference's launchers on device data: a Potential_SOA in device memory whose arrays are
+    // device arrays, a device frame and device planes (src/grid_gradient2_GPU.cu:101-157)
+    Potential_SOA dl{};
+    dl.type = to_device(type);
+    dl.position_x = to_device(arr[0]);
+    dl.position_y = to_device(arr[1]);
+    dl.b0 = to_device(arr[2]);
+    dl.ellipticity_angle = to_device(arr[3]);
+    dl.ellipticity_potential = to_device(arr[4]);
+    dl.rcore = to_device(arr[5]);
+    dl.rcut = to_device(arr[6]);
+    dl.dlsds = to_device(arr[8]);
+    dl.anglecos = to_device(cs);
+    dl.anglesin = to_device(sn);
+    Potential_SOA* lens_gpu = to_device(std::vector<Potential_SOA>{dl});
+    grid_param* frame_gpu = to_device(std::vector<grid_param>{frame});
+    double* dp[4];
+    for (auto& q : dp)
+        if (lt_dev_alloc((void**)&q, (long long)(N * sizeof(double)))) return 6;
+    auto get = [&](int k) {
+        if (lt_memcpy_d2h(p0.data(), dp[k], (long long)(N * sizeof(double)), nullptr)) std::exit(6);
+        put(p0);
+    };
+    module_potentialDerivatives_totalGradient_SOA_CPU_GPU(dp[0], dp[1], frame_gpu, lens_gpu, n, dx, dy, nx, ny, 1, 2);
+    st();
+    get(0);
+    get(1);
+    module_potentialDerivatives_totalGradient2_SOA_CPU_GPU(dp[0], dp[1], dp[2], dp[3], frame_gpu, lens_gpu, n, dx, dy,
+                                                           nx, ny, 1, 2);
+    st();
+    for (int k = 0; k < 4; ++k) get(k);
+    module_potentialDerivatives_Kmap_SOA_CPU_GPU(dp[0], dp[1], dp[2], dp[3], frame_gpu, lens_gpu, n, dx, dy, nx, ny, 0,
+                                                 0);
+    st();
+    get(0);
+    get(2);
+    module_potentialDerivatives_totalGradient_SOA_CPU_GPU(dp[0], dp[1], frame_gpu, lens_gpu, nx, n);
+    st();
+    get(0);
+    module_potentialDerivatives_totalGradient2_SOA_CPU_GPU(dp[0], dp[1], dp[2], dp[3], frame_gpu, lens_gpu, nx, n);
+    st();
+    get(0);
+    module_potentialDerivatives_totalGradient_SOA_CPU_GPU(dp[0], dp[1], frame_gpu, lens_gpu, nx, n);
+    st();
+    rm.dpl = 1;
+    map_gpu_function_t dpl = select_map_dpl_function(&rm);
+    st();
+    if (!dpl) return 7;
+    const double zl = arr[7][0];
+    dpl(dp[0], nullptr, nullptr, nullptr, nullptr, lt_cosmo_object_object(zl, zs, (const lt_cosmo_param*)&cosmo),
+        lt_cosmo_observer_object(zs, (const lt_cosmo_param*)&cosmo), 0.0, cosmo.h, zs, nx, ny, &frame);
+    st();
+    get(0);
+
     FILE* o = std::fopen(argv[2], "wb");
     if (!o) return 5;
     std::fwrite(out.data(), 8, out.size(), o);

@@ -2,14 +2,21 @@
 
 FP64 parity: a GPU value g of a halo sum is accepted when
     |g - x| <= 1e-12 * S + E
-where x is the oracle's extended-precision (80-bit) value of the same formulas, S the sum over
-halos of |per-halo term| (the conditioning of the sum) and E the oracle's estimate of the FP64
-rounding error of the closed form itself, which only matters next to the removable 0/0 point of
-the PIEMD terms (there the reference's own FP64 result is off the exact value by the same amount).
+where x is the oracle's extended-precision (80-bit) value of the same formulas and S the sum over
+halos of |per-halo term| (the conditioning of the sum). E is the error of x itself: the oracle's
+closed form (the reference's) cancels next to the removable 0/0 point of the PIEMD terms, where
+its first-order FP64 rounding estimate `cond` (orc_cond_planes) is large; in 80-bit arithmetic that
+error shrinks by the ratio of the unit roundoffs, 2^-64 / 2^-53 = 2^-11.
+  * The GPU Hessian uses a form without that cancellation, so against the extended oracle it gets
+    E = 2^-10 cond (the oracle's own share, with a factor 2 to spare): EXT_SHARE.
+  * The GPU FP64 gradient evaluates the reference's closed form, as the reference does, and keeps
+    E = cond: its rounding next to the removable point is the reference's own.
+  * Comparisons with numbers of the reference's FP64 build (golden grids) keep E = cond.
 """
 import numpy as np
 
 REL = 1e-12
+EXT_SHARE = 2.0 ** -10
 
 
 def assert_close(name, got, ext, scale, cond, rel=REL):
@@ -30,6 +37,7 @@ def hessian_reference(O, lens, grid, istart=0, jstart=0, kmap=False):
     d = O.hessian_planes(lens, grid, istart, jstart, kmap=kmap, with_abs=True)
     sa, sb, sc = d[4], d[5], d[6]
     _, ch = O.cond_planes(lens, grid, istart, jstart)
+    ch = ch * EXT_SHARE
     if kmap:
         w = float(np.abs(lens.dlsds).max())
         ch = ch * w

@@ -127,3 +127,45 @@ def test_autotune_keeps_results(hip, precision):
         m.close()
     finally:
         hip.stream_destroy(s)
+
+
+def test_memset_is_a_kernel_and_replays(hip):
+    """lt_memset_d sets bytes by a fill kernel: ragged offsets and lengths are set exactly, bytes
+    around them are untouched, and a captured [memset, map] sequence replays correctly (the graph
+    holds no memset node)."""
+    import ctypes as C
+    n = 4099
+    buf = hip.DeviceArray(n, dtype=np.uint8)
+    lib = hip.lib()
+    print("[case] lt_memset_d ragged ranges and a captured memset", flush=True)
+    ref = np.full(n, 0x5A, np.uint8)
+    buf.upload(ref)
+    for off, ln, v in ((0, 1, 1), (1, 15, 2), (3, 40, 3), (17, 1000, 0xFF), (2049, 2050, 0), (4098, 1, 7), (5, 0, 9)):
+        hip.check(lib.lt_memset_d(C.c_void_p(buf.ptr + off), v, ln, None))
+        ref[off:off + ln] = v
+        np.testing.assert_array_equal(buf.download(), ref)
+    buf.close()
+    grid = M.group_grid(96)
+    s = hip.stream_create()
+    try:
+        m = hip.Model(M.group_lens(seed=4), hip.MAP_HESSIAN)
+        out = hip.DeviceArray(grid.npix)
+        flag = hip.DeviceArray(grid.npix)
+        g = hip.Graph(s)
+        hip.check(lib.lt_memset_d(C.c_void_p(flag.ptr), 0, flag.nbytes, C.c_void_p(s)))
+        m.map(hip.MAP_AMPLI, 1, [out.ptr], grid, r=0.7, stream=s)
+        g.end()
+        want = _single(hip, M.group_lens(seed=4), hip.MAP_AMPLI, 1, grid, 0.7, 0, s)
+        for rep in range(3):
+            flag.upload(np.full(grid.npix, 1.5))
+            out.upload(np.zeros(grid.npix))
+            g.launch()
+            hip.sync(s)
+            assert (flag.download() == 0).all()
+            np.testing.assert_array_equal(out.download(), want)
+        g.close()
+        m.close()
+        out.close()
+        flag.close()
+    finally:
+        hip.stream_destroy(s)

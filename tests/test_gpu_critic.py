@@ -5,6 +5,7 @@ count, order), on the real A2744 model with a pitched plane; count-only and over
 agree with it. lt_critic_lines is pinned by closed forms: for a singular isothermal sphere the
 critical line is the circle r = b0 D_ls/D_s and the caustic its centre (both within the linear
 interpolation's O(h^2)); a cored dPIE has an inner radial and an outer tangential line."""
+import ctypes as C
 import math
 import os
 import subprocess
@@ -159,3 +160,33 @@ def test_maps_tool_critical_lines(hip, tmp_path):
     assert np.abs(np.hypot(crit[:, 0], crit[:, 1]) - theta_e).max() < h * h / theta_e + 1e-9
     assert np.hypot(caus[:, 0], caus[:, 1]).max() < h * h / theta_e + 1e-9
     assert os.path.exists(tmp_path / "ampli_sis.fits")
+
+
+def test_external_shear_lines_without_caustics(hip):
+    """Type-14 halos have a Hessian but no deflection: the lines are the contour of the model's own
+    det-A plane, the caustics are NaN and the call reports LT_ERR_TYPE."""
+    cosmo = M.flat_lcdm()
+    zl, zs = 0.3, 2.0
+    lens = M.LensModel.from_halos([{"type": 81, "x": 0.0, "y": 0.0, "sigma": 1000.0, "e": 0.2, "theta": 0.4,
+                                    "rcore": 2.0, "rcut": 300.0},
+                                   {"type": 14, "x": 0.0, "y": 0.0, "b0": 0.0, "epot": 0.03, "theta": 0.7}], zl=zl)
+    n = 300
+    frame = M.grid_param(-60.0, 60.0, -60.0, 60.0)
+    grid = M.Grid(frame, n, n)
+    r = hip.cosmo_dlsds(zl, zs, cosmo)
+    print("[case] dPIE + external shear critical lines 300x300", flush=True)
+    st = hip.lib().lt_critic_lines(C.byref(cosmo), C.byref(frame), C.byref(lens.soa()), lens.n, zl, zs, n, n, 0,
+                                   None, None, None, 0, C.byref(C.c_longlong(0)))
+    assert st == hip.LT_OK  # no caustic array: nothing unavailable
+    crit, caus, kinds, total = hip.critic_lines(lens, frame, n, n, cosmo, zl, zs)
+    assert total == len(crit) > 100 and np.isnan(caus).all()
+    assert b"caustics unavailable" in hip.lib().lt_last_error()
+    kap, det = _fields_device(hip, lens, grid, r, n)
+    try:
+        want, wk, wt, wst = hip.critic_device(det.ptr, kap.ptr, grid, 1 << 16)
+        assert wst == hip.LT_OK and wt == total
+        np.testing.assert_array_equal(crit, want)
+        np.testing.assert_array_equal(kinds, wk)
+    finally:
+        kap.close()
+        det.close()

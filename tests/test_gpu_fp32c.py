@@ -14,6 +14,8 @@ pytestmark = pytest.mark.gpu
 
 DET_TOL = 1e-6
 HESS_TOL = 2e-7
+# FP32C potential: |phi(FP32C) - phi(FP64)| <= POT_REL sum_h |phi_h| at every pixel
+POT_REL = 1e-5
 
 
 @pytest.fixture(scope="module")
@@ -132,3 +134,90 @@ def test_fp32c_cluster_full_size(hip):
     assert err["detA"] <= DET_TOL
     for k in ("kappa", "gamma1", "gamma2"):
         assert err[k] <= HESS_TOL
+
+
+def _pot_map(hip, lens, grid, precision, px=0):
+    m = hip.Model(lens, hip.MAP_POT)
+    out = hip.DeviceArray(grid.npix)
+    m.map(hip.MAP_POT, 3, [out.ptr], grid, precision=precision, px=px)
+    hip.sync()
+    v = out.download().reshape(grid.ny, grid.nx)
+    m.close()
+    out.close()
+    return v
+
+
+@pytest.mark.parametrize("which", ["mixed", "cluster"])
+def test_fp32c_potential(hip, oracle, which):
+    """The FP32C potential against the FP64 kernel (itself checked against the extended oracle in
+    test_gpu_parity.py): the error, relative to the sum over halos of |phi_h| at each pixel, stays
+    below POT_REL; bit-identical for 1, 2 and 4 pixels per thread."""
+    if which == "mixed":
+        lens = _mixed_lens()
+        lens = lens.subset(np.nonzero(lens.type != 14)[0])  # external shear has no potential
+        grid = M.Grid(M.grid_param(-70, 70, -65, 65), 211, 199)
+    else:
+        lens = M.cluster_lens(seed=2)
+        grid = M.Grid(M.grid_param(-150, 150, -150, 150), 160, 150)
+    print_case(f"FP32C potential {which}", lens, grid)
+    f64 = _pot_map(hip, lens, grid, hip.PREC_FP64)
+    f32 = _pot_map(hip, lens, grid, hip.PREC_FP32C)
+    S = np.zeros_like(f64)
+    for i in range(lens.n):
+        S += np.abs(oracle.potential_grid(lens.subset([i]), grid))
+    rel = np.abs(f32 - f64) / S
+    print(f"[fp32c potential {which}] max |d phi| / sum |phi_h| = {rel.max():.3e}, median {np.median(rel):.3e}, "
+          f"max |d phi| = {np.abs(f32 - f64).max():.3e} of max |phi| {np.abs(f64).max():.3e}", flush=True)
+    assert np.isfinite(f32).all()
+    assert rel.max() <= POT_REL
+    for px in (1, 4):
+        np.testing.assert_array_equal(_pot_map(hip, lens, grid, hip.PREC_FP32C, px=px), f32)
+
+
+def _fields_px(hip, model, grid, px, nsamples=1, kind=None):
+    kind = hip.MAP_FIELDS if kind is None else kind
+    outs = [hip.DeviceArray(nsamples * grid.npix) for _ in range(5)]
+    model.map(kind, 0, [o.ptr for o in outs], grid, r=0.7, precision=hip.PREC_FP32C, px=px)
+    hip.sync()
+    v = [o.download().reshape(nsamples, grid.ny, grid.nx) for o in outs]
+    for o in outs:
+        o.close()
+    return v
+
+
+def test_fp32c_halo_lane_shape(hip, oracle):
+    """The lanes-over-halos launch shape of the FP32C Hessian (LT_PX_HALO_LANES): every halo type and a
+    run longer than 64 halos, grid sizes that are not multiples of the 16-pixel segment or the 4-row
+    work-group, a batch of samples; det A within DET_TOL and the Hessian within HESS_TOL of the
+    extended oracle, as the pixel-per-lane shape."""
+    base = _mixed_lens()
+    extra = M.cluster_lens(seed=8, ngal=90)          # a 93-halo dPIE run: two chunks of 64 lanes
+    lens = M.LensModel.from_halos(
+        [dict(type=int(t), x=float(x), y=float(y), b0=float(b), epot=float(e), theta=float(a), rcore=float(rc),
+              rcut=float(rt)) for L in (base, extra)
+         for t, x, y, b, e, a, rc, rt in zip(L.type, L.x, L.y, L.b0, L.epot, L.angle, L.rcore, L.rcut)])
+    grid = M.Grid(M.grid_param(-70.0, 70.0, -65.0, 65.0), 203, 129)
+    print_case("fp32c halo-lane shape", lens, grid)
+    m = hip.Model(lens, hip.MAP_HESSIAN)
+    hl = _fields_px(hip, m, grid, hip.PX_HALO_LANES, kind=hip.MAP_HESSIAN)
+    m.close()
+    ea, eb, ec, _ = oracle.hessian_planes(lens, grid, extended=True)
+    for got, ex in ((hl[0][0], ea), (hl[1][0], eb), (hl[2][0], ec)):
+        assert np.abs(got - ex).max() <= HESS_TOL
+    np.testing.assert_array_equal(hl[1], hl[3])
+    r = 0.7
+    assert np.abs(_detA(hl[0][0], hl[1][0], hl[2][0], r) - _detA(ea, eb, ec, r)).max() <= DET_TOL
+    # a batch of samples: each sample's map equals the single-model launch of the same shape
+    samples = M.perturbed_samples(M.group_lens(seed=4), 3, seed=13)
+    g2 = M.Grid(M.grid_param(-100.0, 100.0, -100.0, 100.0), 97, 61)
+    bm = hip.Model(samples, hip.MAP_HESSIAN)
+    batch = _fields_px(hip, bm, g2, hip.PX_HALO_LANES, nsamples=3)
+    bm.close()
+    for k, L in enumerate(samples):
+        sm = hip.Model(L, hip.MAP_HESSIAN)
+        one = _fields_px(hip, sm, g2, hip.PX_HALO_LANES)
+        sm.close()
+        for p in range(5):
+            np.testing.assert_array_equal(batch[p][k], one[p][0])
+        a, b, c, _ = hip.hessian_grid(L, g2)
+        assert np.abs(batch[3][k] - _detA(a, b, c, 0.7)).max() <= DET_TOL

@@ -12,7 +12,7 @@ import pytest
 
 import ltmodel as M
 from conftest import GOLDEN
-from parity import REL, assert_close, hessian_reference, print_case
+from parity import EXT_SHARE, REL, assert_close, hessian_reference, print_case
 
 pytestmark = pytest.mark.gpu
 
@@ -71,8 +71,10 @@ def test_gradient_mixed(hip, oracle, istart, jstart):
 
 
 def test_gradient_group_lens(hip, oracle):
+    """The group-lens gradient workload at its full size: the whole 512^2 plane (1 dPIE + 10 SIS over
+    200"), every pixel against the oracle."""
     lens = M.group_lens(seed=1)
-    grid = M.group_grid(128)
+    grid = M.group_grid(512)
     print_case("gradient group lens", lens, grid)
     gx, gy = hip.gradient_grid(lens, grid)
     ex, ey = oracle.gradient_grid(lens, grid, extended=True)
@@ -102,10 +104,12 @@ def test_hessian_reference_golden(hip, oracle):
     assert_close("a vs exact", a, ea, np.zeros_like(ta), ta)
     assert_close("b vs exact", b, eb, np.zeros_like(tb), tb)
     assert_close("c vs exact", c, ec, np.zeros_like(tc), tc)
-    # and against the reference's own FP64 numbers: both sit within the tolerance of the exact value
-    assert_close("a vs reference", a, z["a"], np.zeros_like(ta), 2 * ta)
-    assert_close("b vs reference", b, z["b"], np.zeros_like(tb), 2 * tb)
-    assert_close("c vs reference", c, z["c"], np.zeros_like(tc), 2 * tc)
+    # and against the reference's own FP64 numbers: the reference's closed form carries its full FP64
+    # rounding `cond` next to the removable point, the GPU's tolerance on top of it
+    _, ch = oracle.cond_planes(L, grid, i0, j0)
+    assert_close("a vs reference", a, z["a"], np.zeros_like(ta), ta + REL * np.abs(ea) + ch)
+    assert_close("b vs reference", b, z["b"], np.zeros_like(tb), tb + REL * np.abs(eb) + ch)
+    assert_close("c vs reference", c, z["c"], np.zeros_like(tc), tc + REL * np.abs(ec) + ch)
     np.testing.assert_array_equal(b, d)
 
 
@@ -262,6 +266,19 @@ def test_potential(hip, oracle):
     err_gpu = np.abs(phi - ext)
     err_cpu = np.abs(dbl - ext)
     assert (err_gpu <= 1e-10 * scale + 8 * err_cpu).all(), (err_gpu / scale).max()
+    # 1, 2 and 4 pixels per thread give the same map bit for bit
+    m = hip.Model(lens, hip.MAP_POT)
+    out = hip.DeviceArray(grid.npix)
+    maps = []
+    for px in (1, 2, 4):
+        m.map(hip.MAP_POT, 3, [out.ptr], grid, px=px)
+        hip.sync()
+        maps.append(out.download().reshape(grid.ny, grid.nx))
+    m.close()
+    out.close()
+    np.testing.assert_array_equal(maps[0], phi)
+    np.testing.assert_array_equal(maps[1], phi)
+    np.testing.assert_array_equal(maps[2], phi)
 
 
 # ------------------------------------------------------------------------------------------------
@@ -269,9 +286,10 @@ def test_potential(hip, oracle):
 # ------------------------------------------------------------------------------------------------
 
 def test_cluster_fullsize_sampled(hip, oracle):
-    """The cluster Hessian config at full size: the GPU grid at 2000 random pixels equals the oracle's
-    point evaluations there, and rows are independent of the grid they are computed in (a sub-grid
-    with istart/jstart offsets reproduces the same pixels bit for bit)."""
+    """The cluster Hessian workload at full size (4096^2, 503 dPIE): the GPU grid at 2000 random
+    pixels and along whole rows through the 3 cluster-scale cores and 50 galaxy cores equals the
+    extended oracle, and rows are independent of the grid they are computed in (a sub-grid with
+    istart/jstart offsets reproduces the same pixels bit for bit)."""
     lens = M.cluster_lens()
     grid = M.cluster_grid(4096)
     print_case("cluster lens full size", lens, grid)
@@ -287,10 +305,24 @@ def test_cluster_fullsize_sampled(hip, oracle):
         e = oracle.point_hessian(lens, px, py, extended=True)
         s = oracle.point_hessian_abs(lens, px, py)
         _, ch = oracle.point_cond(lens, px, py)
-        tol = REL * max(s[:3]) + ch
+        tol = REL * max(s[:3]) + EXT_SHARE * ch
         err = max(abs(a[rr, cc] - e[0]), abs(b[rr, cc] - e[1]), abs(c[rr, cc] - e[2]))
         worst = max(worst, err / tol)
     assert worst <= 1.0, worst
+    # whole rows through the three cluster-scale cores and 50 galaxy cores, every pixel against the
+    # extended oracle with the FP64 criterion
+    big3 = np.argsort(lens.b0)[-3:]
+    gal = np.setdiff1d(np.arange(lens.n), big3)
+    pick = np.concatenate([big3, rng.choice(gal, 50, replace=False)])
+    rows_sel = sorted({int(round((lens.y[h] - grid.frame.ymin) / grid.dy)) for h in pick
+                       if 0 <= round((lens.y[h] - grid.frame.ymin) / grid.dy) < grid.ny})
+    line = M.Grid(grid.frame, grid.nx, 1, step=(grid.dx, grid.dy))
+    worst = 0.0
+    for j in rows_sel:
+        (ea, eb, ec), (ta, tb, tc) = hessian_reference(oracle, lens, line, 0, j)
+        for name, got, ex, tol in (("a", a, ea, ta), ("b", b, eb, tb), ("c", c, ec, tc)):
+            worst = max(worst, assert_close(f"{name} row {j}", got[j:j + 1], ex, np.zeros_like(tol), tol))
+    assert len(rows_sel) >= 45 and worst <= 1.0
     # sub-grid through istart/jstart
     sub = M.Grid(grid.frame, 300, 200, step=(grid.dx, grid.dy))
     sa, sb, sc, sd = hip.hessian_grid(lens, sub, 1500, 2100)
@@ -317,7 +349,7 @@ def test_real_model_a2744(hip, oracle):
     _, _, _, _, aa, ab, ac = oracle.hessian_planes(lens, grid, with_abs=True)
     _, ch = oracle.cond_planes(lens, grid)
     for got, ex, sc in ((a, ea, aa), (b, eb, ab), (c, ec, ac)):
-        assert (np.abs(got - ex) <= REL * sc + ch).all()
+        assert (np.abs(got - ex) <= REL * sc + EXT_SHARE * ch).all()
     gx, gy = hip.gradient_grid(lens, grid)
     ex, ey = oracle.gradient_grid(lens, grid, extended=True)
     _, _, ax, ay = oracle.gradient_grid(lens, grid, with_abs=True)

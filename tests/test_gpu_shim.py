@@ -40,7 +40,7 @@ def test_reference_signatures(oracle):
     status = struct.unpack("<i", raw[-4:])[0]
     v = np.frombuffer(raw[:-4], dtype="<f8").reshape(-1, n, n)
     assert status == 0
-    assert v.shape[0] == 18
+    assert v.shape[0] == 29
     gx, gy = lthip.gradient_grid(lens, grid)
     np.testing.assert_array_equal(v[0], gx)
     np.testing.assert_array_equal(v[1], gy)
@@ -67,6 +67,18 @@ def test_reference_signatures(oracle):
     pts = v[17].ravel()[:16].reshape(4, 4)
     for k in range(4):
         np.testing.assert_array_equal(pts[k], [H0[0][k, k], H0[1][k, k], H0[2][k, k], H0[3][k, k]])
+    # the launchers on device data give the same planes as the host-array entry points
+    g12 = lthip.gradient_grid(lens, grid, 1, 2)
+    np.testing.assert_array_equal(v[18], g12[0])
+    np.testing.assert_array_equal(v[19], g12[1])
+    for k in range(4):
+        np.testing.assert_array_equal(v[20 + k], H[k])
+    K = lthip.hessian_grid(lens, grid, kmap=True)
+    np.testing.assert_array_equal(v[24], K[0])
+    np.testing.assert_array_equal(v[25], K[2])
+    np.testing.assert_array_equal(v[26], gx)
+    np.testing.assert_array_equal(v[27], H0[0])
+    np.testing.assert_array_equal(v[28], dx)  # dpl mode 1: x * D_ls/D_s, as the DPL map's x plane
     # and the shim's gradient is the oracle's within the FP64 criterion
     ex, ey = oracle.gradient_grid(lens, grid, extended=True)
     _, _, ax, ay = oracle.gradient_grid(lens, grid, with_abs=True)

@@ -265,3 +265,48 @@ def test_cosmology_quadrature(oracle):
     chi = quad(lambda z: 1 / math.sqrt(0.3 * (1 + z) ** 3 + Ok * (1 + z) ** 2), 0, zs, epsabs=1e-15, epsrel=1e-13)[0]
     da = math.sinh(math.sqrt(Ok) * chi) / math.sqrt(Ok) / (1 + zs)
     assert abs(oracle.cosmo_observer_object(zs, c0) - da) < 1e-12
+
+
+def test_epilogue_mode_identities(oracle):
+    """Ampli modes 2, 3, 4, 6 and shear modes 1-4 against quantities computed independently from the
+    matrices: eigenvalues of the Jacobian J = 1 - r H and of H (numpy's symmetric eigensolver), and the
+    spin-2 behaviour of (gamma1, gamma2) when the frame rotates (H' = R^T H R by matrix products)."""
+    rng = np.random.default_rng(11)
+    n = 200
+    a, b, c = rng.normal(0.3, 0.5, n), rng.normal(0, 0.3, n), rng.normal(0.3, 0.5, n)
+    dl0s, dos, dol, h = 0.7, 0.9, 0.5, 1.4
+    r = dl0s / dos
+    H = np.stack([np.stack([a, b], -1), np.stack([b, c], -1)], -2)
+    J = np.eye(2) - r * H
+    lj = np.linalg.eigvalsh(J)            # ascending: mu <= lambda
+    mu_j, lam_j = lj[:, 0], lj[:, 1]
+    lh = np.linalg.eigvalsh(H)
+    one_minus_kappa = 0.5 * (lam_j + mu_j)
+
+    def ep(kind, mode, A=a, B=b, Cc=c):
+        (v,) = oracle.map_epilogue(kind, mode, A, B, Cc, dl0s, dos, dol, h, 0.1, 0.1)
+        return v
+
+    np.testing.assert_allclose(ep(1, 2), 1 / np.abs(lam_j * mu_j), rtol=1e-9)
+    np.testing.assert_allclose(ep(1, 3), -2.5 * np.log10(np.abs(lam_j * mu_j)), rtol=1e-9, atol=1e-12)
+    # mode 4: (1 - kappa)(1 + g^2)/(1 - g^2), g = gamma/(1 - kappa) = (1 - kappa)(lam^2 + mu^2)/(2 lam mu)
+    np.testing.assert_allclose(ep(1, 4), one_minus_kappa * (lam_j ** 2 + mu_j ** 2) / (2 * lam_j * mu_j), rtol=1e-9)
+    # mode 6: the shear without D_ls/D_s = half the spread of the Hessian's eigenvalues
+    np.testing.assert_allclose(ep(1, 6), 0.5 * (lh[:, 1] - lh[:, 0]), rtol=1e-12, atol=1e-14)
+    # shear 1: (lambda - mu) / 2 of J; shear 2: |lam^2 - mu^2| / (lam^2 + mu^2)
+    g = ep(2, 1)
+    np.testing.assert_allclose(g, 0.5 * (lam_j - mu_j), rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(ep(2, 2), np.abs(lam_j ** 2 - mu_j ** 2) / (lam_j ** 2 + mu_j ** 2), rtol=1e-10,
+                               atol=1e-14)
+    # shear 3, 4 = (gamma1, gamma2): |(g1, g2)| = shear 1, and a rotation of the frame by phi turns
+    # (g1, g2) by -2 phi
+    g1, g2 = ep(2, 3), ep(2, 4)
+    np.testing.assert_allclose(np.hypot(g1, g2), g, rtol=1e-12, atol=1e-14)
+    for phi in (0.3, 1.1, -2.0):
+        R = np.array([[np.cos(phi), -np.sin(phi)], [np.sin(phi), np.cos(phi)]])
+        Hr = R.T @ H @ R
+        g1r, g2r = ep(2, 3, Hr[:, 0, 0], Hr[:, 0, 1], Hr[:, 1, 1]), ep(2, 4, Hr[:, 0, 0], Hr[:, 0, 1], Hr[:, 1, 1])
+        np.testing.assert_allclose(g1r, g1 * np.cos(2 * phi) + g2 * np.sin(2 * phi), rtol=1e-10, atol=1e-13)
+        np.testing.assert_allclose(g2r, -g1 * np.sin(2 * phi) + g2 * np.cos(2 * phi), rtol=1e-10, atol=1e-13)
+    # the sign convention of gamma1: a Hessian with a > c has gamma1 > 0
+    assert ep(2, 3, np.array([1.0]), np.array([0.0]), np.array([0.2]))[0] > 0
