@@ -308,6 +308,16 @@ int lt_par_destroy(lt_par par);
  * Waits for the stream. */
 int lt_critic_device(const double* det, const double* kappa, const lt_grid_desc* grid, double* segs, int* kinds,
                      long long max_segments, long long* nsegments, void* stream);
+/* The same contour without allocating: no counterpart in the reference. For callers that cannot
+ * call hipMalloc (code under HIP graph capture, tight serving loops, harnesses that forbid device
+ * allocation). The per-row counts and offsets live in a fixed scratch array of the library's code
+ * object (about 192 KB on each GPU the library runs on, held until the process ends), so grids are
+ * limited to ny - 1 <= LT_CRITIC_NOALLOC_MAX_ROWS cell rows (LT_ERR_CAPACITY beyond). Same
+ * arguments, results and status as lt_critic_device. Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (several host threads or streams) race. */
+#define LT_CRITIC_NOALLOC_MAX_ROWS 16384
+int lt_critic_device_noalloc(const double* det, const double* kappa, const lt_grid_desc* grid, double* segs,
+                             int* kinds, long long max_segments, long long* nsegments, void* stream);
 /* Critical lines of a lens for the source plane z_source on an nx x ny grid over `frame` (det A
  * and kappa from the fields map in `precision`, then lt_critic_device), and optionally the
  * caustics: each segment end theta mapped to beta = theta - (D_ls/D_s) alpha(theta), alpha the

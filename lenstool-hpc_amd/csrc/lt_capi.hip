@@ -309,7 +309,7 @@ int lt_init(int device)
     if (device < 0 || device >= n || device >= kMaxDevices) return fail(LT_ERR_INVALID, "device index out of range");
     HIP_TRY(hipSetDevice(device));
     g_device = device;
-    return LT_OK;
+    return critic_noalloc_setup(device, nullptr);
 }
 
 int lt_sync(void)

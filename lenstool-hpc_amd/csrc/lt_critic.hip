@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -263,29 +264,22 @@ namespace {
 
 int fail(int code, const std::string& msg) { return set_error(code, msg); }
 
-}  // namespace
-
-extern "C" {
-
-int lt_critic_device(const double* det, const double* kappa, const lt_grid_desc* grid, double* segs, int* kinds,
-                     long long max_segments, long long* nsegments, void* stream)
+// Row counts (cell rows, padded to 256 bytes) followed by nrows + 1 row offsets.
+size_t critic_scratch_bytes(int nrows)
 {
-    if (!det || !grid || !nsegments) return fail(LT_ERR_INVALID, "NULL argument");
-    if (grid->nx < 2 || grid->ny < 2 || grid->pitch < grid->nx)
-        return fail(LT_ERR_INVALID, "a contour needs at least 2 x 2 pixels and pitch >= nx");
-    if (kinds && !kappa) return fail(LT_ERR_INVALID, "segment kinds need the kappa plane");
-    if (segs && max_segments < 0) return fail(LT_ERR_INVALID, "negative max_segments");
-    if (!std::isfinite(grid->dx) || !std::isfinite(grid->dy)) return fail(LT_ERR_INVALID, "non-finite step");
-    int dev, st;
-    if ((st = current_device(&dev))) return st;
-    hipStream_t s = (hipStream_t)stream;
+    return (((size_t)nrows * sizeof(int) + 255) & ~(size_t)255) + (size_t)(nrows + 1) * sizeof(long long);
+}
+
+// The launches of both critic entry points on a caller-provided scratch buffer of
+// critic_scratch_bytes(ny - 1) bytes; every scratch element the kernels read is written first, so
+// the buffer needs no clearing. Waits for the stream.
+int critic_run(const double* det, const double* kappa, const lt_grid_desc* grid, double* segs, int* kinds,
+               long long max_segments, long long* nsegments, hipStream_t s, void* buf)
+{
     const int nrows = grid->ny - 1;
     CritGrid g{grid->xmin, grid->ymin, grid->dx, grid->dy, grid->nx, grid->ny, grid->istart, grid->jstart, grid->pitch};
-    void* scratch = nullptr;
-    const size_t cnt_bytes = ((size_t)nrows * sizeof(int) + 255) & ~(size_t)255;
-    LT_HIP_TRY(hipMalloc(&scratch, cnt_bytes + (size_t)(nrows + 1) * sizeof(long long)));
-    int* row_count = (int*)scratch;
-    long long* row_off = (long long*)((char*)scratch + cnt_bytes);
+    int* row_count = (int*)buf;
+    long long* row_off = (long long*)((char*)buf + (((size_t)nrows * sizeof(int) + 255) & ~(size_t)255));
     long long total = 0;
     hipError_t e = hipSuccess;
     hipLaunchKernelGGL(k_critic_count, dim3(nrows), dim3(kCritThreads), 0, s, det, g, row_count);
@@ -302,12 +296,94 @@ int lt_critic_device(const double* det, const double* kappa, const lt_grid_desc*
     }
     const hipError_t es = hipStreamSynchronize(s);
     if (e == hipSuccess) e = es;
-    (void)hipFree(scratch);
     if (e != hipSuccess) return hip_error(e, "critical lines");
     *nsegments = total;
     if (segs && total > max_segments)
         return fail(LT_ERR_OVERFLOW, std::to_string(total) + " segments, room for " + std::to_string(max_segments));
     return LT_OK;
+}
+
+int critic_check(const double* det, const double* kappa, const lt_grid_desc* grid, double* segs, int* kinds,
+                 long long max_segments, long long* nsegments)
+{
+    if (!det || !grid || !nsegments) return fail(LT_ERR_INVALID, "NULL argument");
+    if (grid->nx < 2 || grid->ny < 2 || grid->pitch < grid->nx)
+        return fail(LT_ERR_INVALID, "a contour needs at least 2 x 2 pixels and pitch >= nx");
+    if (kinds && !kappa) return fail(LT_ERR_INVALID, "segment kinds need the kappa plane");
+    if (segs && max_segments < 0) return fail(LT_ERR_INVALID, "negative max_segments");
+    if (!std::isfinite(grid->dx) || !std::isfinite(grid->dy)) return fail(LT_ERR_INVALID, "non-finite step");
+    return LT_OK;
+}
+
+}  // namespace
+
+// Scratch of lt_critic_device_noalloc: a fixed array in the code object (each GPU holds its own copy
+// for the life of the process). Host code never uses the array's name as an address: its device
+// address on each GPU is taken once with hipGetSymbolAddress and kept in g_noalloc_ptr.
+constexpr int kNoAllocRows = LT_CRITIC_NOALLOC_MAX_ROWS;
+constexpr size_t kNoAllocBytes = (((size_t)kNoAllocRows * sizeof(int) + 255) & ~(size_t)255) +
+                                 (size_t)(kNoAllocRows + 1) * sizeof(long long);
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kMaxDevices] = {};
+
+}  // namespace
+
+namespace lt {
+
+// With `dev` current: look up (once per device) the device address of the no-alloc scratch array.
+int critic_noalloc_setup(int dev, void** ptr)
+{
+    if (dev < 0 || dev >= kMaxDevices) return set_error(LT_ERR_NO_DEVICE, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return hip_error(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(critic scratch)");
+        }
+        g_noalloc_ptr[dev] = p;
+    }
+    if (ptr) *ptr = g_noalloc_ptr[dev];
+    return LT_OK;
+}
+
+}  // namespace lt
+
+extern "C" {
+
+int lt_critic_device(const double* det, const double* kappa, const lt_grid_desc* grid, double* segs, int* kinds,
+                     long long max_segments, long long* nsegments, void* stream)
+{
+    int st = critic_check(det, kappa, grid, segs, kinds, max_segments, nsegments);
+    if (st) return st;
+    int dev;
+    if ((st = current_device(&dev))) return st;
+    void* buf = nullptr;
+    LT_HIP_TRY(hipMalloc(&buf, critic_scratch_bytes(grid->ny - 1)));
+    st = critic_run(det, kappa, grid, segs, kinds, max_segments, nsegments, (hipStream_t)stream, buf);
+    (void)hipFree(buf);
+    return st;
+}
+
+int lt_critic_device_noalloc(const double* det, const double* kappa, const lt_grid_desc* grid, double* segs,
+                             int* kinds, long long max_segments, long long* nsegments, void* stream)
+{
+    int st = critic_check(det, kappa, grid, segs, kinds, max_segments, nsegments);
+    if (st) return st;
+    if (grid->ny - 1 > kNoAllocRows)
+        return fail(LT_ERR_CAPACITY, std::to_string(grid->ny) + " rows: the no-alloc contour takes at most " +
+                                         std::to_string(kNoAllocRows + 1));
+    int dev;
+    if ((st = current_device(&dev))) return st;
+    void* ptr = nullptr;
+    if ((st = critic_noalloc_setup(dev, &ptr))) return st;
+    return critic_run(det, kappa, grid, segs, kinds, max_segments, nsegments, (hipStream_t)stream, ptr);
 }
 
 int lt_critic_lines(const lt_cosmo_param* cosmo, const lt_grid_param* frame, const lt_potential_soa* lens, int nhalos,

@@ -17,7 +17,9 @@ int current_device(int* dev);
 hipError_t launch_fill_i32(int32_t* p, int32_t v, size_t n, hipStream_t s);
 // fill `bytes` bytes of device memory with v (kernels only: no memset node when captured)
 hipError_t launch_fill_bytes(void* dst, unsigned char v, size_t bytes, hipStream_t s);
-
+// with `dev` current: the device address of lt_critic_device_noalloc's scratch array on `dev`, looked
+// up once per device (ptr may be NULL: set-up only)
+int critic_noalloc_setup(int dev, void** ptr);
 
 }  // namespace lt
 

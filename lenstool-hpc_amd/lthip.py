@@ -136,6 +136,8 @@ SIGNATURES = {
     "lt_par_destroy": ([C.c_void_p], C.c_int),
     "lt_critic_device": ([C.c_void_p, C.c_void_p, C.POINTER(LtGridDesc), C.c_void_p, C.c_void_p, C.c_longlong,
                           C.POINTER(C.c_longlong), C.c_void_p], C.c_int),
+    "lt_critic_device_noalloc": ([C.c_void_p, C.c_void_p, C.POINTER(LtGridDesc), C.c_void_p, C.c_void_p,
+                                  C.c_longlong, C.POINTER(C.c_longlong), C.c_void_p], C.c_int),
     "lt_critic_lines": ([C.POINTER(M.LtCosmoParam), C.POINTER(M.LtGridParam), C.POINTER(M.LtPotentialSOA), C.c_int,
                          C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
                          C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_longlong, C.POINTER(C.c_longlong)], C.c_int),
@@ -567,8 +569,10 @@ class ParFile:
             pass
 
 
-def critic_device(det_ptr, kappa_ptr, grid: M.Grid, max_segments, stream=0, istart=0, jstart=0, pitch=None):
-    """Contour det A = 0 of device planes; returns (segments (n, 4), kinds or None, total, status)."""
+def critic_device(det_ptr, kappa_ptr, grid: M.Grid, max_segments, stream=0, istart=0, jstart=0, pitch=None,
+                  noalloc=False):
+    """Contour det A = 0 of device planes; returns (segments (n, 4), kinds or None, total, status).
+    noalloc: through lt_critic_device_noalloc (fixed scratch array) instead of lt_critic_device."""
     g = LtGridDesc(grid.frame.xmin, grid.frame.ymin, grid.dx, grid.dy, grid.nx, grid.ny, istart, jstart,
                    pitch or grid.nx)
     total = C.c_longlong(0)
@@ -577,7 +581,8 @@ def critic_device(det_ptr, kappa_ptr, grid: M.Grid, max_segments, stream=0, ista
     dsegs = DeviceArray(4 * room) if room > 0 else None
     dkinds = DeviceArray(room, dtype=np.int32) if (kappa_ptr and room > 0) else None
     try:
-        st = lib().lt_critic_device(C.c_void_p(det_ptr), C.c_void_p(kappa_ptr) if dkinds is not None else None,
+        fn = lib().lt_critic_device_noalloc if noalloc else lib().lt_critic_device
+        st = fn(C.c_void_p(det_ptr), C.c_void_p(kappa_ptr) if dkinds is not None else None,
                                     C.byref(g), C.c_void_p(dsegs.ptr) if dsegs is not None else None,
                                     C.c_void_p(dkinds.ptr) if dkinds is not None else None,
                                     room, C.byref(total), C.c_void_p(stream))
