@@ -381,6 +381,7 @@ constexpr int SEG = 16;
 constexpr int FLUSH = 8;  // chunks of 64 halos between two FP64 flushes: <= 8 FP32 terms per lane sum
 
 // one FP32 term of halo record h at lab-frame offsets (tx, row part rw), as (T, D, B) in the lab frame
+// with the halo's amplitudes applied
 template <int TYPE>
 __device__ __forceinline__ void term32(const HaloRec32& h, const Row32& rw, float tx, float& T, float& D, float& B)
 {
@@ -462,10 +463,9 @@ __device__ __forceinline__ void run32h(const HaloRec* __restrict__ h64, const Ha
             float T, D, B;
             term32<TYPE>(h, rw, tx, T, D, B);
             const bool keep = !(__builtin_fmaf(tx, tx, rw.ty2) < h.near2) && live;
-            T = keep ? T : 0.f;
-            D = keep ? D : 0.f;
-            B = keep ? B : 0.f;
-            acc_lab(h, T, D, B, aT[j], aD[j], aB[j]);
+            aT[j] += keep ? T : 0.f;
+            aD[j] += keep ? D : 0.f;
+            aB[j] += keep ? B : 0.f;
         }
         if (++pending == FLUSH) {
             flush_lanes(lds, aT, aD, aB, lane, sT, sD, sB);
@@ -783,10 +783,13 @@ hipError_t launch_gradient32(const HaloRec* h64, const HaloRec32* recs32, const 
 // hardware base-2 logarithm; the offset pixel - centre formed in FP64 and rounded once; FP32 terms
 // summed per chunk of CHUNK halos into FP64 run sums. Every term is FP32 (the potential has no
 // singular core: it is finite and smooth at the centre of a cored halo), and the map reports its
-// error against the FP64 kernel.
+// error against the FP64 kernel. Floating-point contraction is off in these functions: only the
+// written FMAs are fused, so that every launch shape (pixels per thread) evaluates the same operations
+// and gives the same map.
 // ------------------------------------------------------------------------------------------------
 __device__ __forceinline__ void log_ratio32(float ur, float ui, float vr, float vi, float& lr, float& li)
 {
+#pragma clang fp contract(off)
     lr = 0.5f * logf_lt(__builtin_fmaf(ur, ur, ui * ui) * rcp32(__builtin_fmaf(vr, vr, vi * vi)));
     li = atan2f_lt(__builtin_fmaf(ui, vr, -(ur * vi)), __builtin_fmaf(ur, vr, ui * vi));
 }
@@ -797,6 +800,7 @@ struct PotShared32 {
 
 __device__ __forceinline__ float pi05_core32(const PotShared32& s, float x, float y, float rc, float irc, float rc2)
 {
+#pragma clang fp contract(off)
     const float W2 = rc2 + s.rem2;
     const float W = W2 * rsq32(W2);
     const float onePC = 1.f + s.PC;
@@ -814,6 +818,7 @@ __device__ __forceinline__ float pi05_core32(const PotShared32& s, float x, floa
 template <int TYPE>
 __device__ __forceinline__ float pot32(const PotRec32& h, float tx, float ty)
 {
+#pragma clang fp contract(off)
     const float x = __builtin_fmaf(tx, h.cs, ty * h.sn);
     const float y = __builtin_fmaf(ty, h.cs, -(tx * h.sn));
     const float x2 = x * x, y2 = y * y;
@@ -845,6 +850,7 @@ template <int TYPE, int PX>
 __device__ __forceinline__ void runp32(const HaloRec* __restrict__ h64, const PotRec32* __restrict__ p32, int start,
                                        int end, const double* px, double py, double* s)
 {
+#pragma clang fp contract(off)
     for (int c0 = start; c0 < end; c0 += CHUNK) {
         const int c1 = min(c0 + CHUNK, end);
         float f[PX];
@@ -855,7 +861,10 @@ __device__ __forceinline__ void runp32(const HaloRec* __restrict__ h64, const Po
             const double xh = h64[i].xh;
             const float ty = (float)(py - h64[i].yh);
 #pragma unroll
-            for (int k = 0; k < PX; ++k) f[k] += pot32<TYPE>(h, offset_x(xh, px[k]), ty);
+            for (int k = 0; k < PX; ++k) {
+                const float v = pot32<TYPE>(h, offset_x(xh, px[k]), ty);
+                f[k] = f[k] + v;
+            }
         }
 #pragma unroll
         for (int k = 0; k < PX; ++k) s[k] += (double)f[k];

@@ -37,6 +37,7 @@ def lib():
         L.orc_point_kmap.argtypes = [P(M.LtMatrix), P(M.LtPoint), soa, C.c_int]
         L.orc_point_potential.argtypes = [dp, P(M.LtPoint), soa, C.c_int]
         L.orc_point_gradient2_abs.argtypes = [P(M.LtMatrix), P(M.LtPoint), soa, C.c_int]
+        L.orc_point_gradient_abs.argtypes = [P(M.LtPoint), P(M.LtPoint), soa, C.c_int]
         L.orc_gradient_grid.argtypes = [dp, dp, gp, soa, C.c_int, C.c_double, C.c_double,
                                         C.c_int, C.c_int, C.c_int, C.c_int]
         L.orc_gradient_abs_planes.argtypes = [dp, dp, gp, soa, C.c_int, C.c_double, C.c_double,
@@ -138,6 +139,14 @@ def set_threads(n: int):
 
 def get_threads() -> int:
     return int(lib().orc_get_threads())
+
+
+def point_gradient_abs(lens: M.LensModel, x: float, y: float):
+    """Sum over halos of |gradient term| (x, y): the conditioning of the gradient sum at a point."""
+    p = M.LtPoint(x, y)
+    out = M.LtPoint()
+    _chk(lib().orc_point_gradient_abs(C.byref(out), C.byref(p), C.byref(lens.soa()), lens.n), "gradient abs")
+    return out.x, out.y
 
 
 def point_gradient(lens: M.LensModel, x: float, y: float, extended=False):

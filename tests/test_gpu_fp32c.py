@@ -165,10 +165,14 @@ def test_fp32c_potential(hip, oracle, which):
     S = np.zeros_like(f64)
     for i in range(lens.n):
         S += np.abs(oracle.potential_grid(lens.subset([i]), grid))
-    rel = np.abs(f32 - f64) / S
-    print(f"[fp32c potential {which}] max |d phi| / sum |phi_h| = {rel.max():.3e}, median {np.median(rel):.3e}, "
-          f"max |d phi| = {np.abs(f32 - f64).max():.3e} of max |phi| {np.abs(f64).max():.3e}", flush=True)
-    assert np.isfinite(f32).all()
+    # the I_0.5 potential is 0/0 at a halo centre (the reference's gives NaN there too): both kernels
+    # give NaN at exactly the same pixels, and are compared everywhere else
+    fin = np.isfinite(f64)
+    np.testing.assert_array_equal(np.isfinite(f32), fin)
+    rel = np.abs(f32 - f64)[fin] / S[fin]
+    print(f"[fp32c potential {which}] {(~fin).sum()} centre pixels; max |d phi| / sum |phi_h| = {rel.max():.3e}, "
+          f"median {np.median(rel):.3e}, max |d phi| = {np.abs(f32 - f64)[fin].max():.3e} of max |phi| "
+          f"{np.abs(f64[fin]).max():.3e}", flush=True)
     assert rel.max() <= POT_REL
     for px in (1, 4):
         np.testing.assert_array_equal(_pot_map(hip, lens, grid, hip.PREC_FP32C, px=px), f32)
