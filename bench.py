@@ -333,7 +333,7 @@ def serve_sweep(pool, rates, requests, slots=8, budget_gb=64, seed=9, until_satu
         mi = int(rng.integers(len(pool)))
         grid = M.Grid(M.grid_param(-150.0, 150.0, -150.0, 150.0), n, n)
         p = int(rng.integers(2))
-        return mi, k, m, grid, p if k != lthip.MAP_POT else 0
+        return mi, k, m, grid, p
 
     rows = []
     todo = list(rates)
@@ -358,16 +358,25 @@ def serve_sweep(pool, rates, requests, slots=8, budget_gb=64, seed=9, until_satu
         th = threading.Thread(target=producer)
         t0 = time.perf_counter()
         th.start()
+        lat = np.empty(len(reqs))
         for i in range(len(reqs)):
             ready.acquire()
             _, k, _, grid, _ = reqs[i]
-            srv.fetch(tickets[i], k, grid.npix)
+            _, tm = srv.fetch(tickets[i], k, grid.npix)
+            lat[i] = tm.done_ms - tm.submit_ms
         th.join()
         wall = time.perf_counter() - t0
         st = srv.stats()
+        # latency by request size: halos x pixels, the quarter of smallest and the tenth of largest requests
+        work = np.array([pool[mi].n * grid.npix for mi, _, _, grid, _ in reqs], dtype=float)
+        small, large = work <= np.quantile(work, 0.25), work >= np.quantile(work, 0.9)
         row = {"offered_req_per_s": rate, "achieved_req_per_s": len(reqs) / wall, "requests": len(reqs),
                "wall_s": wall, "halo_px_per_s": st["halo_px"] / wall, "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
-               "p50_fetch_ms": st["p50_fetch_ms"], "p99_fetch_ms": st["p99_fetch_ms"], "bytes_peak": st["bytes_peak"]}
+               "p50_fetch_ms": st["p50_fetch_ms"], "p99_fetch_ms": st["p99_fetch_ms"], "bytes_peak": st["bytes_peak"],
+               "smallest_quarter": {"halo_px_max": float(work[small].max()), "p50_ms": float(np.median(lat[small])),
+                                    "p99_ms": float(np.quantile(lat[small], 0.99))},
+               "largest_tenth": {"halo_px_min": float(work[large].min()), "p50_ms": float(np.median(lat[large])),
+                                 "p99_ms": float(np.quantile(lat[large], 0.99))}}
         rows.append(row)
         print(f"[serve] {json.dumps(row)}", file=sys.stderr, flush=True)
         if until_saturation and not todo and row["achieved_req_per_s"] >= 0.9 * rate:

@@ -70,7 +70,9 @@ struct alignas(64) HaloRec32 {
     float near2;                        // 17
     float sa, sb;                       // 18, 19  SIS: 1 - e, 1 + e
     float circ;                         // 20      1 for a circular SIS
-    float pad[11];                      // 21..31
+    float near2s;                       // 21      near bound on X^2 + Y^2 (scaled frame; lab frame for circular SIS)
+    float nearr;                        // 22      radius of a disk about the centre holding that near region
+    float pad[9];                       // 23..31
 };
 static_assert(sizeof(HaloRec32) == 128, "fp32 halo record is 128 bytes");
 

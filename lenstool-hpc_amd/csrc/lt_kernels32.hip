@@ -368,31 +368,102 @@ static hipError_t launch_hessian32_px(const HaloRec* h64, const HaloRec32* h32, 
 // ------------------------------------------------------------------------------------------------
 // The same FP32C Hessian with lanes over halos (launch shape LT_PX_HALO_LANES): a wavefront takes a
 // row segment of SEG pixels and walks each run 64 halos at a time, lane l holding halo c0 + l, so
-// every halo constant of the FP32 terms is a VGPR operand (on gfx950 an FP32 instruction with an
-// SGPR operand issues at half the rate of an all-VGPR one) and the near-core test is a compare and
-// select per (halo, pixel), with no branch. Each lane keeps FP32 sums of its halos' terms for the SEG
-// pixels (at most FLUSH halos per sum, as the chunks of the pixel-per-lane kernel); they are summed
-// over the 64 lanes in FP64 through LDS into the FP64 run sums of lane j = pixel j. The (halo, pixel)
-// pairs inside a halo's near radius are masked out of the FP32 sums and added from the FP64 terms,
-// lane j = pixel j, for the halos whose near disk reaches the segment (the same FP32 near test, so
-// each pair is taken exactly once). Same epilogue and det-A guard as k_hessian32.
+// every operand of the FP32 terms is a VGPR (on gfx950 an FP32 instruction with an SGPR operand
+// issues at half the rate of an all-VGPR one). Per lane and row, the halo-frame coordinates of the
+// segment's first pixel (X0, Y0: the FP64 offset rounded once to FP32, then rotated and scaled) and
+// their step per pixel (dX, dY) are formed once; pixel j is at (X0 + j dX, Y0 + j dY). A (halo,
+// pixel) pair is near when X^2 + Y^2 < near2s (a region holding the disk of the halo's near radius):
+// near pairs are masked out of the FP32 sums by a select, and a second pass, lane j = pixel j, adds
+// their FP64 terms for the halos whose near region reaches the segment, evaluating the same FP32
+// test with the same operations, so each pair is taken exactly once. Each lane keeps FP32 sums of
+// its halos' terms for the SEG pixels (at most FLUSH halos per sum, as the chunks of k_hessian32);
+// they are summed over the 64 lanes in FP64 through LDS into the FP64 run sums of lane j = pixel j.
+// Same epilogue and det-A guard as k_hessian32.
 // ------------------------------------------------------------------------------------------------
 constexpr int SEG = 16;
 constexpr int FLUSH = 8;  // chunks of 64 halos between two FP64 flushes: <= 8 FP32 terms per lane sum
+static_assert(WAVE == 4 * SEG, "the flush adds four quarters of the wavefront");
 
-// one FP32 term of halo record h at lab-frame offsets (tx, row part rw), as (T, D, B) in the lab frame
-// with the halo's amplitudes applied
+// one lane's halo over the segment: halo-frame coordinates of pixel 0 and their step per pixel (for
+// a circular SIS: the lab-frame x offset and its step, the y offset and its square)
+struct Seg32 {
+    float X0, Y0, dX, dY, ty2;
+};
+
 template <int TYPE>
-__device__ __forceinline__ void term32(const HaloRec32& h, const Row32& rw, float tx, float& T, float& D, float& B)
+__device__ __forceinline__ Seg32 seg32(const HaloRec32& h, float tx0, float ty, float dxf)
 {
-    T = D = B = 0.f;
+    Seg32 s;
     if constexpr (TYPE == TYPE_SIS_CIRC) {
-        hess_sis32_circ(h, tx, rw.ty, rw.ty2, T, D, B);
+        s.X0 = tx0;
+        s.Y0 = ty;
+        s.dX = dxf;
+        s.dY = 0.f;
+        s.ty2 = ty * ty;
     } else {
-        const Frame32 f = frame32(h, rw, tx);
-        if constexpr (TYPE == TYPE_DPIE) hess_piemd32<true>(h, f, T, D, B);
-        else if constexpr (TYPE == TYPE_PIEMD) hess_piemd32<false>(h, f, T, D, B);
-        else hess_sis32(h, f, T, D, B);
+        s.X0 = __builtin_fmaf(tx0, h.csx, ty * h.snx);
+        s.Y0 = __builtin_fmaf(-tx0, h.sny, ty * h.csy);
+        s.dX = dxf * h.csx;
+        s.dY = -(dxf * h.sny);
+        s.ty2 = 0.f;
+    }
+    return s;
+}
+
+// pixel J of the segment and its near metric: the same operations in the FP32 pass and the near pass
+template <int TYPE>
+__device__ __forceinline__ float seg_x(const Seg32& s, float J)
+{
+    return __builtin_fmaf(J, s.dX, s.X0);
+}
+
+template <int TYPE>
+__device__ __forceinline__ float seg_y(const Seg32& s, float J)
+{
+    if constexpr (TYPE == TYPE_SIS_CIRC) return s.Y0;
+    else return __builtin_fmaf(J, s.dY, s.Y0);
+}
+
+template <int TYPE>
+__device__ __forceinline__ float near_metric(const Seg32& s, float X, float Y)
+{
+    if constexpr (TYPE == TYPE_SIS_CIRC) return __builtin_fmaf(X, X, s.ty2);
+    else return __builtin_fmaf(Y, Y, X * X);
+}
+
+// the FP32 term of a lane's halo at pixel (X, Y) into the lane sums, in the lab frame with the halo's
+// amplitudes; a near pair adds exactly zero (the select drops whatever the FP32 form gave there)
+template <int TYPE>
+__device__ __forceinline__ void add_term32(const HaloRec32& h, const Seg32& s, float X, float Y, bool far, float& aT,
+                                           float& aD, float& aB)
+{
+    if constexpr (TYPE == TYPE_SIS_CIRC) {
+        const float x2 = X * X;
+        const float iR = far ? rsq32(x2 + s.ty2) : 0.f;
+        const float ga = h.hT * (iR * iR * iR);
+        aT = __builtin_fmaf(h.hT, iR, aT);
+        aD = __builtin_fmaf(ga, s.ty2 - x2, aD);
+        aB = __builtin_fmaf(-ga, X * Y, aB);
+    } else if constexpr (TYPE == TYPE_SIS) {
+        const float x2 = X * X, y2 = Y * Y;
+        const float iR = rsq32(__builtin_fmaf(h.sa, x2, h.sb * y2));
+        const float g = far ? iR * iR * iR : 0.f;
+        acc_lab(h, g * (x2 + y2), g * (y2 - x2), -(g * X * Y), aT, aD, aB);
+    } else {
+        const QShared q = q_shared(h, X, Y);
+        const Q32 a = q_term(h, q, h.ra, h.ra2c);
+        float xx, xy, yy;
+        if constexpr (TYPE == TYPE_DPIE) {
+            const Q32 b = q_term(h, q, h.rb, h.rb2c);
+            xx = __builtin_fmaf(a.xx, a.iq, -(b.xx * b.iq));
+            xy = __builtin_fmaf(a.xy, a.iq, -(b.xy * b.iq));
+            yy = __builtin_fmaf(a.yy, a.iq, -(b.yy * b.iq));
+        } else {
+            xx = a.xx * a.iq;
+            xy = a.xy * a.iq;
+            yy = a.yy * a.iq;
+        }
+        acc_lab(h, far ? xx + yy : 0.f, far ? xx - yy : 0.f, far ? xy : 0.f, aT, aD, aB);
     }
 }
 
@@ -405,20 +476,31 @@ __device__ __forceinline__ void term64(const HaloRec& h, double px, double py, d
     else hess_sis(h, px, py, sT, sD, sB);
 }
 
-// sum over the 64 lanes of one component's lane sums a[j] (j < SEG) into pixel lane j's FP64 sum s, in
-// lane order, through this wavefront's LDS slice (layout [source lane][pixel])
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// sum over the 64 lanes of one component's lane sums a[j] (j < SEG) into pixel lane j's FP64 sum s,
+// through this wavefront's LDS slice (layout [source lane][pixel]): lane (p = lane % SEG,
+// q = lane / SEG) adds the sums of source lanes 16 q .. 16 q + 15 for pixel p in FP64, then the four
+// quarters are added (q + 2 into q, then 1 into 0). A fixed order: the same sums on every run.
 __device__ __forceinline__ void flush_component(float* __restrict__ lds, const float (&a)[SEG], int lane, double& s)
 {
+    float4* w = reinterpret_cast<float4*>(lds + lane * SEG);
 #pragma unroll
-    for (int j = 0; j < SEG; ++j) lds[lane * SEG + j] = a[j];
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    if (lane < SEG)
-        for (int l = 0; l < WAVE; ++l) s += (double)lds[l * SEG + lane];
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int k = 0; k < SEG / 4; ++k) w[k] = make_float4(a[4 * k], a[4 * k + 1], a[4 * k + 2], a[4 * k + 3]);
+    wave_sync();
+    const int p = lane & (SEG - 1), q = lane / SEG;
+    double part = 0.;
+#pragma unroll
+    for (int l = 0; l < SEG; ++l) part += (double)lds[(q * SEG + l) * SEG + p];
+    wave_sync();
+    part += __shfl_down(part, 2 * SEG);
+    part += __shfl_down(part, SEG);
+    if (lane < SEG) s += part;
 }
 
 __device__ __forceinline__ void flush_lanes(float* __restrict__ lds, float (&aT)[SEG], float (&aD)[SEG],
@@ -431,10 +513,17 @@ __device__ __forceinline__ void flush_lanes(float* __restrict__ lds, float (&aT)
     for (int j = 0; j < SEG; ++j) aT[j] = aD[j] = aB[j] = 0.f;
 }
 
+// the segment: pixel 0 at px0, pixels 0 .. SEG-1 spanning [xa, xb] (FP64), step dxf (FP32); lane j
+// (j < SEG) owns pixel j at pxl
+struct SegPix {
+    double px0, xa, xb, pxl, py, slack;
+    float dxf;
+};
+
 template <int TYPE>
 __device__ __forceinline__ void run32h(const HaloRec* __restrict__ h64, const HaloRec32* __restrict__ h32, int start,
-                                       int end, const double (&pxs)[SEG], double pxl, double xa, double xb, double py,
-                                       int lane, float* __restrict__ lds, double& sT, double& sD, double& sB)
+                                       int end, const SegPix& sp, int lane, float* __restrict__ lds, double& sT,
+                                       double& sD, double& sB)
 {
     float aT[SEG], aD[SEG], aB[SEG];
 #pragma unroll
@@ -450,37 +539,41 @@ __device__ __forceinline__ void run32h(const HaloRec* __restrict__ h64, const Ha
             xh = h64[i].xh;
             yh = h64[i].yh;
         } else {
-            h.near2 = -1.f;  // never near, never a candidate; its amplitudes are zero
+            h.near2s = 3.0e38f;  // every pair near: the lane adds nothing and is never a candidate
         }
-        const Row32 rw = row32(h, yh, py);
-        // near disk reaching the segment [xa, xb] x {py} (FP64, with a margin over the FP32 test)
-        const double dxs = __builtin_fmax(0., __builtin_fmax(xa - xh, xh - xb));
-        const double dys = py - yh;
-        const bool cand = live && __builtin_fma(dxs, dxs, dys * dys) < (double)h.near2 * (1. + 1e-5) + 1e-30;
+        const Seg32 sg = seg32<TYPE>(h, (float)(sp.px0 - xh), (float)(sp.py - yh), sp.dxf);
+        // does the halo's near region (inside the disk of radius nearr) reach the segment? FP64, with
+        // a margin well above the FP32 rounding of the near test
+        const double dxs = __builtin_fmax(0., __builtin_fmax(sp.xa - xh, xh - sp.xb));
+        const double dys = sp.py - yh;
+        const double rr = (double)h.nearr * 1.01 + sp.slack;
+        const bool cand = live && __builtin_fma(dxs, dxs, dys * dys) < rr * rr;
 #pragma unroll
         for (int j = 0; j < SEG; ++j) {
-            const float tx = offset_x(xh, pxs[j]);
-            float T, D, B;
-            term32<TYPE>(h, rw, tx, T, D, B);
-            const bool keep = !(__builtin_fmaf(tx, tx, rw.ty2) < h.near2) && live;
-            aT[j] += keep ? T : 0.f;
-            aD[j] += keep ? D : 0.f;
-            aB[j] += keep ? B : 0.f;
+            const float J = (float)j;
+            const float X = seg_x<TYPE>(sg, J), Y = seg_y<TYPE>(sg, J);
+            const bool far = !(near_metric<TYPE>(sg, X, Y) < h.near2s);
+            add_term32<TYPE>(h, sg, X, Y, far, aT[j], aD[j], aB[j]);
         }
         if (++pending == FLUSH) {
             flush_lanes(lds, aT, aD, aB, lane, sT, sD, sB);
             pending = 0;
         }
-        // FP64 terms of the near pairs, candidate halos in order, lane j = pixel j
+        // FP64 terms of the near pairs, candidate halos in lane order, lane j = pixel j
         uint64_t mask = __builtin_amdgcn_ballot_w64(cand);
         while (mask) {
             const int c = __builtin_ctzll(mask);
             mask &= mask - 1;
-            const double cxh = __shfl(xh, c);
-            const float cty2 = __shfl(rw.ty2, c);
-            const float cnear2 = __shfl(h.near2, c);
-            const float tx = offset_x(cxh, pxl);
-            if (lane < SEG && __builtin_fmaf(tx, tx, cty2) < cnear2) term64<TYPE>(h64[c0 + c], pxl, py, sT, sD, sB);
+            Seg32 cs;
+            cs.X0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, sg.X0), c));
+            cs.Y0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, sg.Y0), c));
+            cs.dX = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, sg.dX), c));
+            cs.dY = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, sg.dY), c));
+            cs.ty2 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, sg.ty2), c));
+            const float cn2 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, h.near2s), c));
+            const float J = (float)lane;
+            const float X = seg_x<TYPE>(cs, J), Y = seg_y<TYPE>(cs, J);
+            if (lane < SEG && near_metric<TYPE>(cs, X, Y) < cn2) term64<TYPE>(h64[c0 + c], sp.pxl, sp.py, sT, sD, sB);
         }
     }
     if (pending) flush_lanes(lds, aT, aD, aB, lane, sT, sD, sB);
@@ -492,7 +585,7 @@ __global__ __launch_bounds__(WAVE* ROWS) void k_hessian32h(const HaloRec* __rest
                                                            const Run* __restrict__ runs, int nruns, GridDesc g,
                                                            OutDesc o)
 {
-    __shared__ float lds_all[ROWS][WAVE * SEG];
+    __shared__ __align__(16) float lds_all[ROWS][WAVE * SEG];
     const int row = blockIdx.y * ROWS + threadIdx.y;
     if (row >= g.ny) return;  // the whole wavefront: no work-group barrier is used
     float* lds = lds_all[threadIdx.y];
@@ -500,13 +593,17 @@ __global__ __launch_bounds__(WAVE* ROWS) void k_hessian32h(const HaloRec* __rest
     h32 += (int64_t)blockIdx.z * g.halo_stride;
     const int lane = threadIdx.x;
     const int col0 = blockIdx.x * SEG;
-    const double py = pixel_coord(g.ymin, row + g.jstart, g.dy);
-    // this lane's pixel (lanes 0 .. SEG-1), and all SEG pixel coordinates (wavefront-uniform)
-    const double pxl = pixel_coord(g.xmin, col0 + (lane < SEG ? lane : 0) + g.istart, g.dx);
-    double pxs[SEG];
-#pragma unroll
-    for (int j = 0; j < SEG; ++j) pxs[j] = pixel_coord(g.xmin, col0 + j + g.istart, g.dx);
-    const double xa = pxs[0], xb = pxs[SEG - 1];
+    SegPix sp;
+    sp.py = pixel_coord(g.ymin, row + g.jstart, g.dy);
+    sp.px0 = pixel_coord(g.xmin, col0 + g.istart, g.dx);
+    const double pxe = pixel_coord(g.xmin, col0 + SEG - 1 + g.istart, g.dx);
+    sp.xa = __builtin_fmin(sp.px0, pxe);
+    sp.xb = __builtin_fmax(sp.px0, pxe);
+    sp.pxl = pixel_coord(g.xmin, col0 + (lane < SEG ? lane : 0) + g.istart, g.dx);
+    sp.dxf = (float)g.dx;
+    // the FP32 rounding of the near test moves a pixel by at most about 2^-22 (2 d + 2 SEG |dx|), d its
+    // distance to the centre: the candidate test's 1 % and this slack cover it
+    sp.slack = 1e-4 + 1e-6 * (double)SEG * __builtin_fabs(g.dx);
     double tT = 0., tD = 0., tB = 0.;
     for (int r = 0; r < nruns; ++r) {
         const Run run = runs[r];
@@ -520,13 +617,13 @@ __global__ __launch_bounds__(WAVE* ROWS) void k_hessian32h(const HaloRec* __rest
                 sB += h.h14b;
             }
         } else if (run.type == TYPE_DPIE) {
-            run32h<TYPE_DPIE>(h64, h32, run.start, end, pxs, pxl, xa, xb, py, lane, lds, sT, sD, sB);
+            run32h<TYPE_DPIE>(h64, h32, run.start, end, sp, lane, lds, sT, sD, sB);
         } else if (run.type == TYPE_PIEMD) {
-            run32h<TYPE_PIEMD>(h64, h32, run.start, end, pxs, pxl, xa, xb, py, lane, lds, sT, sD, sB);
+            run32h<TYPE_PIEMD>(h64, h32, run.start, end, sp, lane, lds, sT, sD, sB);
         } else if (run.circ) {
-            run32h<TYPE_SIS_CIRC>(h64, h32, run.start, end, pxs, pxl, xa, xb, py, lane, lds, sT, sD, sB);
+            run32h<TYPE_SIS_CIRC>(h64, h32, run.start, end, sp, lane, lds, sT, sD, sB);
         } else {
-            run32h<TYPE_SIS>(h64, h32, run.start, end, pxs, pxl, xa, xb, py, lane, lds, sT, sD, sB);
+            run32h<TYPE_SIS>(h64, h32, run.start, end, sp, lane, lds, sT, sD, sB);
         }
         tT = __builtin_fma(run.weight, sT, tT);
         tD = __builtin_fma(run.weight, sD, tD);
@@ -537,7 +634,7 @@ __global__ __launch_bounds__(WAVE* ROWS) void k_hessian32h(const HaloRec* __rest
         if constexpr (KIND == LT_MAP_FIELDS || KIND == LT_MAP_AMPLI || KIND == LT_MAP_SHEAR) {
             const double kap = 0.5 * tT * o.r;
             const double gb = 0.5 * (__builtin_fabs(tD) + 2. * __builtin_fabs(tB)) * __builtin_fabs(o.r);
-            if (nruns > 0 && __builtin_fabs(1. - kap) + gb > kDetGuard) hessian_fp64_pixel(h64, runs, nruns, pxl, py, tT, tD, tB);
+            if (nruns > 0 && __builtin_fabs(1. - kap) + gb > kDetGuard) hessian_fp64_pixel(h64, runs, nruns, sp.pxl, sp.py, tT, tD, tB);
         }
         hess_epilogue<KIND>(o, (int64_t)row * g.pitch + col, tT, tD, tB);
     }

@@ -174,6 +174,10 @@ int pack_model(const lt_potential_soa* L, int n, Use use, std::vector<HaloRec>& 
             const double t05 = (t == TYPE_DPIE) ? h.rcut / (h.rcut - h.rc) : 1.0;
             const double R = kFp32NearScalePiemd * std::fabs(b0 * t05);
             f.near2 = (float)std::fmin(R * R, 3.0e38);
+            // X^2 + Y^2 = cx1^2 (a^2 x^2 + b^2 y^2) <= (cx1 b)^2 d^2 = a^2 d^2 (b > a): the region
+            // X^2 + Y^2 < a^2 R^2 holds the disk d < R and lies inside d < R / cx1
+            f.near2s = (float)std::fmin(a * a * R * R, 3.0e38);
+            f.nearr = (float)std::fmin(R * (1.0 + eps) / (1.0 - eps), 1.0e19);
         } else if (t == TYPE_SIS) {
             f.sa = (float)h.sa;
             f.sb = (float)h.sb;
@@ -186,8 +190,12 @@ int pack_model(const lt_potential_soa* L, int n, Use use, std::vector<HaloRec>& 
             f.circ = (float)h.circ;
             const double R = kFp32NearScaleSis * std::fabs(b0);
             f.near2 = (float)(R * R);
+            f.near2s = f.near2;  // plain rotation (and the lab frame for a circular SIS): X^2 + Y^2 = d^2
+            f.nearr = (float)R;
         } else {
             f.near2 = 3.0e38f;  // type 14 stays FP64 (a constant per halo)
+            f.near2s = 3.0e38f;
+            f.nearr = 1.0e19f;
         }
         if (use == Use::Gradient) {
             // gradient models carry the gradient's FP32 constants instead (same near radius)
