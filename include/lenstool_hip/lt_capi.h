@@ -223,7 +223,8 @@ int lt_server_add_model(lt_server server, const lt_potential_soa* lens, int nhal
 int lt_server_submit(lt_server server, const lt_request* req, long long* ticket);
 /* Wait for a request (timeout_ms < 0: no time limit); the request's status if it failed. LT_ERR_BUSY
  * on timeout, and also, whatever the timeout, when the request cannot be admitted until results
- * submitted before it are fetched and no work or fetch is under way (the wait would never end). */
+ * submitted before it are fetched and no work or fetch is under way (the wait would never end).
+ * LT_ERR_NOT_FOUND if the ticket is unknown or is fetched by another thread during the wait. */
 int lt_server_wait(lt_server server, long long ticket, double timeout_ms);
 /* Wait, copy the request's planes to the given host arrays (NULL = skip a plane), release the
  * ticket and its device memory. Plane order as lt_map_device. LT_ERR_BUSY as lt_server_wait (the

@@ -472,6 +472,10 @@ int lt_server_wait(lt_server s, long long ticket, double timeout_ms)
         }
         if (timeout_ms >= 0 && Clock::now() >= deadline) return set_error(LT_ERR_BUSY, "not finished");
         s->cv_done.wait_for(lk, std::chrono::milliseconds(1));
+        // another thread may have fetched (and so released) the ticket while this one slept
+        it = s->tickets.find(ticket);
+        if (it == s->tickets.end()) return set_error(LT_ERR_NOT_FOUND, "ticket released while waiting");
+        t = it->second.get();
     }
     if (t->state == State::Failed) return set_error(t->status, t->err);
     return LT_OK;

@@ -175,6 +175,33 @@ def test_wait_that_could_never_end_is_busy(hip):
         srv.close()
 
 
+def test_wait_while_another_thread_fetches(hip):
+    """A thread waiting on a ticket that another thread fetches returns (LT_OK if it saw the request
+    finish first, LT_ERR_NOT_FOUND if the ticket was released during its wait) instead of touching
+    the released ticket; the fetch gets the result."""
+    import ctypes as C
+    import threading
+    pool = _pool()
+    g = M.Grid(M.grid_param(-100.0, 100.0, -100.0, 100.0), 1024, 1024)
+    srv = hip.Server(nslots=2, hbm_budget_bytes=1 << 30)
+    try:
+        mid = srv.add_model(pool[2])
+        print("[case] server wait on a ticket fetched by another thread", flush=True)
+        for _ in range(4):
+            t = srv.submit(mid, hip.MAP_POT, 1, g, r=0.7)
+            got = []
+            th = threading.Thread(target=lambda: got.append(hip.lib().lt_server_wait(srv.h, t, C.c_double(-1.0))))
+            th.start()
+            out, _ = srv.fetch(t, hip.MAP_POT, g.npix)
+            th.join(timeout=60)
+            assert not th.is_alive()
+            assert got[0] in (0, 8)  # LT_OK or LT_ERR_NOT_FOUND
+            assert np.isfinite(out[0]).sum() > 0
+            assert hip.lib().lt_server_wait(srv.h, t, C.c_double(0.0)) == 8
+    finally:
+        srv.close()
+
+
 def test_serving_workload_sizes(hip):
     """Requests at the serving workload's sizes: 8192^2 fields (kappa, gamma, det A, mu) of a
     2000-halo model, 8192^2 potential of a 2000-halo model, 8192^2 FP32C amplification, and small
