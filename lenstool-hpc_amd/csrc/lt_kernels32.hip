@@ -874,42 +874,55 @@ hipError_t launch_gradient32(const HaloRec* h64, const HaloRec32* recs32, const 
 }
 
 // ------------------------------------------------------------------------------------------------
-// FP32-compensated potential: the algebraic I_{0.5} potential of lt_terms64.h (pot_halo: two
-// logarithms and two arguments per core radius instead of the reference's complex cosh / sinh / log,
-// src/potential_GPU.cu:47-116) evaluated in FP32 with the single-precision arctangent and the
-// hardware base-2 logarithm; the offset pixel - centre formed in FP64 and rounded once; FP32 terms
+// FP32-compensated potential: the algebraic I_{0.5} potential of lt_terms64.h (pot_halo: both
+// logarithms as log1p of quantities that vanish at the centre, the second ones of the two radii
+// merged into one, instead of the reference's complex cosh / sinh / log,
+// src/potential_GPU.cu:47-116) evaluated in FP32 with the single-precision arctangent, a series
+// log1p and the hardware base-2 logarithm; the offset pixel - centre formed in FP64 and rounded once; FP32 terms
 // summed per chunk of CHUNK halos into FP64 run sums. Every term is FP32 (the potential has no
 // singular core: it is finite and smooth at the centre of a cored halo), and the map reports its
 // error against the FP64 kernel. Floating-point contraction is off in these functions: only the
 // written FMAs are fused, so that every launch shape (pixels per thread) evaluates the same operations
 // and gives the same map.
 // ------------------------------------------------------------------------------------------------
-__device__ __forceinline__ void log_ratio32(float ur, float ui, float vr, float vi, float& lr, float& li)
-{
-#pragma clang fp contract(off)
-    lr = 0.5f * logf_lt(__builtin_fmaf(ur, ur, ui * ui) * rcp32(__builtin_fmaf(vr, vr, vi * vi)));
-    li = atan2f_lt(__builtin_fmaf(ui, vr, -(ur * vi)), __builtin_fmaf(ur, vr, ui * vi));
-}
-
 struct PotShared32 {
-    float p, q, PC, pq, Nr, Ni, rem2, isrem, srem;
+    float p, q, PC, pq, Nr, Ni, rem2, isrem, srem, onePC, kr, ki;
 };
 
-__device__ __forceinline__ float pi05_core32(const PotShared32& s, float x, float y, float rc, float irc, float rc2)
+// ln(1 + a) in FP32 for a > -1: 2 atanh(s), s = a / (2 + a), by its series through s^9 for
+// -0.3 < a < 0.4 (|s| < 0.18: the next term is below 4e-9 relative), the hardware logarithm of 1 + a
+// elsewhere (where 1 + a is not close to 1)
+__device__ __forceinline__ float log1pf_lt(float a)
+{
+#pragma clang fp contract(off)
+    const float sv = a * rcp32(2.f + a);
+    const float w = sv * sv;
+    float p = __builtin_fmaf(w, 1.f / 9.f, 1.f / 7.f);
+    p = __builtin_fmaf(w, p, 1.f / 5.f);
+    p = __builtin_fmaf(w, p, 1.f / 3.f);
+    const float two_s = sv + sv;
+    const float series = __builtin_fmaf(two_s * w, p, two_s);
+    return (a > -0.3f && a < 0.4f) ? series : logf_lt(1.f + a);
+}
+
+// the core-radius part: a1 = -ln(1 + z), z = (rem / rc) t / D, t = rem / (rc + W) (lt_terms64.h
+// pot_core)
+struct PotCore32 {
+    float a1r, a1i, t;
+};
+
+__device__ __forceinline__ PotCore32 pot_core32(const PotShared32& s, float rc, float irc, float rc2)
 {
 #pragma clang fp contract(off)
     const float W2 = rc2 + s.rem2;
     const float W = W2 * rsq32(W2);
-    const float onePC = 1.f + s.PC;
-    float a1r, a1i, a2r, a2i;
-    log_ratio32(onePC, -s.pq, __builtin_fmaf(W, irc, s.PC), -s.pq, a1r, a1i);
-    const float t = s.srem * rcp32(rc + W);
-    log_ratio32(__builtin_fmaf(t, s.Nr, onePC), __builtin_fmaf(t, s.Ni, -s.pq), __builtin_fmaf(-t, s.Nr, onePC),
-                __builtin_fmaf(-t, s.Ni, -s.pq), a2r, a2i);
-    const float sz = s.srem * irc;
-    const float cr = __builtin_fmaf(s.Nr, a1r, -(s.Ni * a1i)) + sz * a2r;
-    const float ci = __builtin_fmaf(s.Nr, a1i, s.Ni * a1r) + sz * a2i;
-    return rc * s.isrem * __builtin_fmaf(ci, x, -(cr * y));
+    PotCore32 c;
+    c.t = s.srem * rcp32(rc + W);
+    const float w1 = (s.srem * irc) * c.t;
+    const float zr = w1 * s.kr, zi = w1 * s.ki;
+    c.a1r = -0.5f * log1pf_lt(__builtin_fmaf(zi, zi, zr * (2.f + zr)));
+    c.a1i = -atan2f_lt(zi, 1.f + zr);
+    return c;
 }
 
 template <int TYPE>
@@ -936,10 +949,42 @@ __device__ __forceinline__ float pot32(const PotRec32& h, float tx, float ty)
         s.rem2 = __builtin_fmaf(x2, h.kx, y2 * h.ky);
         s.isrem = rsq32(s.rem2);
         s.srem = s.rem2 * s.isrem;
-        const float pa = pi05_core32(s, x, y, h.ra, h.ira, h.ra2);
-        if constexpr (TYPE == TYPE_PIEMD) return h.amp * pa;
-        const float ps = pi05_core32(s, x, y, h.rb, h.irb, h.rb2);
-        return h.amp * (pa - ps);
+        s.onePC = 1.f + s.PC;
+        const float iD2 = rcp32(__builtin_fmaf(s.onePC, s.onePC, s.pq * s.pq));
+        s.kr = s.onePC * iD2;
+        s.ki = s.pq * iD2;
+        const PotCore32 a = pot_core32(s, h.ra, h.ira, h.ra2);
+        // a2 = ln(1 + z2), z2 = U / V (lt_terms64.h pot_halo)
+        const float amr = __builtin_fmaf(-a.t, s.Nr, s.onePC), ami = __builtin_fmaf(-a.t, s.Ni, -s.pq);
+        float cr, ci, Ur, Ui, Vr, Vi;
+        if constexpr (TYPE == TYPE_PIEMD) {
+            Ur = 2.f * a.t * s.Nr;
+            Ui = 2.f * a.t * s.Ni;
+            Vr = amr;
+            Vi = ami;
+            cr = h.ra * a.a1r;
+            ci = h.ra * a.a1i;
+        } else {
+            const PotCore32 b = pot_core32(s, h.rb, h.irb, h.rb2);
+            const float bpr = __builtin_fmaf(b.t, s.Nr, s.onePC), bpi = __builtin_fmaf(b.t, s.Ni, -s.pq);
+            Vr = __builtin_fmaf(amr, bpr, -(ami * bpi));
+            Vi = __builtin_fmaf(amr, bpi, ami * bpr);
+            const float dt = 2.f * (a.t - b.t);
+            const float dnr = __builtin_fmaf(s.onePC, s.Nr, s.pq * s.Ni);
+            const float dni = __builtin_fmaf(s.onePC, s.Ni, -(s.pq * s.Nr));
+            Ur = dt * dnr;
+            Ui = dt * dni;
+            cr = __builtin_fmaf(h.ra, a.a1r, -(h.rb * b.a1r));
+            ci = __builtin_fmaf(h.ra, a.a1i, -(h.rb * b.a1i));
+        }
+        const float iV2 = rcp32(__builtin_fmaf(Vr, Vr, Vi * Vi));
+        const float z2r = __builtin_fmaf(Ur, Vr, Ui * Vi) * iV2, z2i = __builtin_fmaf(Ui, Vr, -(Ur * Vi)) * iV2;
+        const float a2r = 0.5f * log1pf_lt(__builtin_fmaf(z2i, z2i, z2r * (2.f + z2r)));
+        const float a2i = atan2f_lt(z2i, 1.f + z2r);
+        const float nr = __builtin_fmaf(s.Nr, cr, -(s.Ni * ci)), ni = __builtin_fmaf(s.Nr, ci, s.Ni * cr);
+        const float l1 = __builtin_fmaf(ni, x, -(nr * y));
+        const float l2 = __builtin_fmaf(a2i, x, -(a2r * y));
+        return h.amp * __builtin_fmaf(s.isrem, l1, l2);
     }
 }
 

@@ -176,6 +176,19 @@ __device__ __forceinline__ double log_lt(double x)
     return __builtin_fma(de, kLn2Hi, __builtin_fma(de, kLn2Lo, lm));
 }
 
+// ln(1 + a) in FP64 for finite a > -1, within a few units in the last place also for small |a|:
+// x = 1 + a rounded, then ln(1 + a) = ln x + ln(1 + d / x) ~ ln x + d / x with d = a - (x - 1) the
+// rounding error of x (exact for -1/2 <= a <= 1: x - 1 and a - (x - 1) are then differences of
+// numbers within a factor 2 of each other, Sterbenz; outside that range ln x is not small and d / x
+// is below its last place); the hardware reciprocal is enough for the correction term.
+// log_lt itself is accurate near 1 (its m - 1 is exact there).
+__device__ __forceinline__ double log1p_lt(double a)
+{
+    const double x = 1. + a;
+    const double d = a - (x - 1.);
+    return __builtin_fma(d, __builtin_amdgcn_rcp(x), log_lt(x));
+}
+
 // Pixel coordinate exactly as the reference CPU forms it: xmin + (i * d), two roundings, no fused
 // multiply-add (src/grid_gradient_CPU.cpp:89-90).
 __device__ __forceinline__ double pixel_coord(double xmin, int i, double d)

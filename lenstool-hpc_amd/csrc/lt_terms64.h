@@ -244,11 +244,23 @@ __device__ __forceinline__ void grad_sis_circ(const HaloRec& h, double px, doubl
 // (1 + tanh a tanh b) / (1 - tanh a tanh b))
 //   ln(cosh^2 eta / (cosh(eta+zeta) cosh(eta-zeta))) = ln((1 + P C - i p q) / (P C + W / rc - i p q))
 //   ln(cosh(eta+zeta) / cosh(eta-zeta))             = ln((D + t N) / (D - t N)),
-// N = -p C + i P q, D = 1 + P C - i p q, t = tanh zeta. Two logarithms and two arguments per core
-// radius instead of five complex cosh / sinh, two complex logarithms, asinh and asin. The parts that
-// depend only on the pixel and the ellipticity are shared by the core and cut radii.
+// N = -p C + i P q, D = 1 + P C - i p q, t = tanh zeta. Two more steps make it shorter and
+// accurate next to the centre:
+//  * the first logarithm is -ln(1 + z) with z = (W / rc - 1) / D and W / rc - 1 = rem^2 / (rc (rc + W))
+//    = (rem / rc) t, formed without cancellation; ln|1 + z| = log1p(2 Re z + |z|^2) / 2. (Written as
+//    the logarithm of a quotient of two nearly equal numbers, its relative error grows as rc^2 / r^2
+//    at a distance r from the centre: about 1e-11 of the potential at r = rc / 100.)
+//  * the second logarithm a2 = ln((D + t N) / (D - t N)) = ln(1 + z2), z2 = 2 t N / (D - t N), is
+//    also taken as a log1p (its quotient tends to 1 at the centre, where t -> 0);
+//  * for the dPIE, the second logarithms of the core and cut radii enter the difference with the same
+//    weight, so they are one logarithm: ln((D + ta N)(D - tb N) / ((D - ta N)(D + tb N))) = ln(1 + z2)
+//    with z2 = 2 (ta - tb) D N / ((D - ta N)(D + tb N)). Its argument is the difference of the two
+//    arguments (no branch cut is crossed: each arg cosh(eta +- zeta) = atan(tanh(a +- zeta) tan b)
+//    lies in (-pi/4, pi/4), eta = a + i b, |b| < pi/4, so each argument difference is in
+//    (-pi/2, pi/2) and theirs in (-pi, pi)).
+// Per dPIE halo x pixel: three logarithms and three arguments instead of four of each.
 struct PotShared {
-    double p, q, PC, pq, Nr, Ni, rem2, isrem, srem;
+    double p, q, PC, pq, Nr, Ni, rem2, isrem, srem, onePC, kr, ki;
 };
 
 // pixel quantities shared by both core radii (per-halo constants from the record: the divisions
@@ -268,44 +280,74 @@ __device__ __forceinline__ PotShared pot_shared(const HaloRec& h, const PixelFra
     s.rem2 = __builtin_fma(f.x2, h.kx, f.y2 * h.ky);
     s.isrem = rsq64(s.rem2);
     s.srem = s.rem2 * s.isrem;
+    // 1 / D = conj(D) / |D|^2 = (kr + i ki) with D = 1 + PC - i pq
+    s.onePC = 1. + s.PC;
+    const double iD2 = rcp64(__builtin_fma(s.onePC, s.onePC, s.pq * s.pq));
+    s.kr = s.onePC * iD2;
+    s.ki = s.pq * iD2;
     return s;
 }
 
-// ln(u / v) and arg(u / v) of two complex numbers: one logarithm, one atan2
-__device__ __forceinline__ void log_ratio(double ur, double ui, double vr, double vi, double& lr, double& li)
-{
-    lr = 0.5 * log_lt(__builtin_fma(ur, ur, ui * ui) * rcp64(__builtin_fma(vr, vr, vi * vi)));
-    li = atan2_lt(__builtin_fma(ui, vr, -(ur * vi)), __builtin_fma(ur, vr, ui * vi));
-}
+// The core-radius part of pi05: a1 = ln(cosh^2 eta / (cosh(eta+zeta) cosh(eta-zeta))) = -ln(1 + z),
+// z = (rem / rc) t / D, and t = tanh zeta = rem / (rc + W).
+struct PotCore {
+    double a1r, a1i, t;
+};
 
-// pi05 of one core radius rc (irc = 1 / rc) without the amplitude b0 ci:
-// rc / rem ((c1 + c2).im x - (c1 + c2).re y)
-__device__ __forceinline__ double pi05_core(const PotShared& s, double x, double y, double rc, double irc)
+__device__ __forceinline__ PotCore pot_core(const PotShared& s, double rc, double irc)
 {
     const double W2 = __builtin_fma(rc, rc, s.rem2);
     const double W = W2 * rsq64(W2);
-    const double onePC = 1. + s.PC;
-    double a1r, a1i, a2r, a2i;
-    log_ratio(onePC, -s.pq, __builtin_fma(W, irc, s.PC), -s.pq, a1r, a1i);
-    const double t = s.srem * rcp64(rc + W);
-    log_ratio(__builtin_fma(t, s.Nr, onePC), __builtin_fma(t, s.Ni, -s.pq), __builtin_fma(-t, s.Nr, onePC),
-              __builtin_fma(-t, s.Ni, -s.pq), a2r, a2i);
-    // c1 = sinh(2 eta) a1 with sinh 2 eta = N; c2 = (rem / rc) a2
-    const double sz = s.srem * irc;
-    const double cr = __builtin_fma(s.Nr, a1r, -(s.Ni * a1i)) + sz * a2r;
-    const double ci = __builtin_fma(s.Nr, a1i, s.Ni * a1r) + sz * a2i;
-    return rc * s.isrem * __builtin_fma(ci, x, -(cr * y));
+    PotCore c;
+    c.t = s.srem * rcp64(rc + W);
+    const double w1 = (s.srem * irc) * c.t;
+    const double zr = w1 * s.kr, zi = w1 * s.ki;
+    c.a1r = -0.5 * log1p_lt(__builtin_fma(zi, zi, zr * (2. + zr)));
+    c.a1i = -atan2_lt(zi, 1. + zr);
+    return c;
 }
 
+// pi05 without the amplitude b0 ci, as rc / rem L(N a1) + L(a2) with L(c) = c.im x - c.re y:
+// the reference's rc / rem ((c1 + c2).im x - (c1 + c2).re y), c1 = sinh(2 eta) a1 = N a1,
+// c2 = (rem / rc) a2; for the dPIE the difference of the core and cut radii's values.
 __device__ __forceinline__ double pot_halo(const HaloRec& h, int type, double px, double py)
 {
     PixelFrame f = halo_frame(h, px, py);
     if (type == TYPE_SIS) return h.b0 * sqrt(__builtin_fma(h.sa, f.x2, h.sb * f.y2));
     const PotShared s = pot_shared(h, f);
-    const double pa = pi05_core(s, f.x, f.y, h.rc, h.irc);
-    if (type == TYPE_PIEMD) return h.ampg * pa;
-    const double ps = pi05_core(s, f.x, f.y, h.rcut, h.ircut);
-    return h.ampg * (pa - ps);
+    const PotCore a = pot_core(s, h.rc, h.irc);
+    // a2 = ln(1 + z2), z2 = U / V: PIEMD U = 2 t N, V = D - t N; dPIE U = 2 (ta - tb) D N,
+    // V = (D - ta N)(D + tb N)
+    const double amr = __builtin_fma(-a.t, s.Nr, s.onePC), ami = __builtin_fma(-a.t, s.Ni, -s.pq);
+    double cr, ci, Ur, Ui, Vr, Vi;
+    if (type == TYPE_PIEMD) {
+        Ur = 2. * a.t * s.Nr;
+        Ui = 2. * a.t * s.Ni;
+        Vr = amr;
+        Vi = ami;
+        cr = h.rc * a.a1r;
+        ci = h.rc * a.a1i;
+    } else {
+        const PotCore b = pot_core(s, h.rcut, h.ircut);
+        const double bpr = __builtin_fma(b.t, s.Nr, s.onePC), bpi = __builtin_fma(b.t, s.Ni, -s.pq);
+        Vr = __builtin_fma(amr, bpr, -(ami * bpi));
+        Vi = __builtin_fma(amr, bpi, ami * bpr);
+        const double dt = 2. * (a.t - b.t);
+        const double dnr = __builtin_fma(s.onePC, s.Nr, s.pq * s.Ni), dni = __builtin_fma(s.onePC, s.Ni, -(s.pq * s.Nr));
+        Ur = dt * dnr;
+        Ui = dt * dni;
+        cr = __builtin_fma(h.rc, a.a1r, -(h.rcut * b.a1r));
+        ci = __builtin_fma(h.rc, a.a1i, -(h.rcut * b.a1i));
+    }
+    const double iV2 = rcp64(__builtin_fma(Vr, Vr, Vi * Vi));
+    const double z2r = __builtin_fma(Ur, Vr, Ui * Vi) * iV2, z2i = __builtin_fma(Ui, Vr, -(Ur * Vi)) * iV2;
+    const double a2r = 0.5 * log1p_lt(__builtin_fma(z2i, z2i, z2r * (2. + z2r)));
+    const double a2i = atan2_lt(z2i, 1. + z2r);
+    // N (cr + i ci)
+    const double nr = __builtin_fma(s.Nr, cr, -(s.Ni * ci)), ni = __builtin_fma(s.Nr, ci, s.Ni * cr);
+    const double l1 = __builtin_fma(ni, f.x, -(nr * f.y));
+    const double l2 = __builtin_fma(a2i, f.x, -(a2r * f.y));
+    return h.ampg * __builtin_fma(s.isrem, l1, l2);
 }
 
 // ------------------------------------------------------------------------------------------------

@@ -254,6 +254,28 @@ def test_map_fields_and_dpl(hip, oracle):
     np.testing.assert_allclose(dy, gy * r, rtol=1e-15, atol=0)
 
 
+@pytest.mark.parametrize("htype", [81, 8])
+def test_potential_next_to_the_centre(hip, oracle, htype):
+    """Pixels 1e-3 to 3e-2 core radii from a halo centre (never on it): the first logarithm of the
+    algebraic potential is formed as log1p of a small quantity, so the 1e-12 criterion holds there too
+    (written as the logarithm of a quotient of nearly equal numbers it lost up to 1e-8 of the
+    potential at such pixels)."""
+    halos = [dict(type=htype, x=0.37, y=-0.21, sigma=420.0, e=0.35, theta=0.6, rcore=0.5, rcut=60.0),
+             dict(type=81, x=8.0, y=5.0, sigma=250.0, e=0.2, theta=1.9, rcore=0.2, rcut=30.0)]
+    lens = M.LensModel.from_halos(halos)
+    n, step = 33, 1e-3
+    x0, y0 = 0.37 - (n // 2 + 0.5) * step, -0.21 - (n // 2 + 0.5) * step
+    grid = M.Grid(M.grid_param(x0, x0 + (n - 1) * step, y0, y0 + (n - 1) * step), n, n)
+    print_case(f"potential next to a type-{htype} centre", lens, grid)
+    phi = hip.potential_grid(lens, grid)
+    ext = oracle.potential_grid(lens, grid, extended=True)
+    S = np.zeros_like(ext)
+    for i in range(lens.n):
+        S += np.abs(oracle.potential_grid(lens.subset([i]), grid, extended=True))
+    assert np.isfinite(phi).all()
+    assert_close("phi", phi, ext, S, np.zeros_like(S))
+
+
 @pytest.mark.parametrize("which", ["mixed", "cluster"])
 def test_potential(hip, oracle, which):
     """The potential (the algebraic form of the reference's pi05) against the extended oracle of the
