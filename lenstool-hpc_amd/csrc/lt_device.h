@@ -25,9 +25,9 @@ struct alignas(64) HaloRec {
     double hc2, hs2;                // 4, 5
     double kx, ky;                  // 6, 7   1/(1+e)^2, 1/(1-e)^2
     double e2;                      // 8      2 sqrt(e)
-    double cx1, icx1, cx1sq;        // 9..11  (1-e)/(1+e), its inverse, its square
+    double cx1, gki, gkv;           // 9..11  (1-e)/(1+e); gradient: 2 sqrt(e) a^2, 2 sqrt(e) a b (a, b = 1/(1 +- e))
     double ra2, rb2;                // 12, 13 rcore^2, rcut^2
-    double e2ra, e2rb;              // 14, 15 2 rcore sqrt(e), 2 rcut sqrt(e)
+    double rca, rcb;                // 14, 15 cx1 rcore, cx1 rcut
     double ampg;                    // 16     gradient amplitude
     double amph;                    // 17     Hessian amplitude
     double csx, snx, csy, sny;      // 18..21 cx1 a (cs, sn), cx1 b (cs, sn): the scaled Hessian frame

@@ -100,12 +100,12 @@ int pack_model(const lt_potential_soa* L, int n, Use use, std::vector<HaloRec>& 
             h.ky = 1.0 / ((1.0 - eps) * (1.0 - eps));
             h.e2 = 2.0 * sqe;
             h.cx1 = cx1;
-            h.icx1 = 1.0 / cx1;
-            h.cx1sq = cx1 * cx1;
+            h.gki = 2.0 * sqe / ((1.0 + eps) * (1.0 + eps));
+            h.gkv = 2.0 * sqe / ((1.0 + eps) * (1.0 - eps));
             h.ra2 = rc * rc;
             h.rb2 = rcut * rcut;
-            h.e2ra = 2.0 * rc * sqe;
-            h.e2rb = 2.0 * rcut * sqe;
+            h.rca = cx1 * rc;
+            h.rcb = cx1 * rcut;
             const double t05 = (t == TYPE_DPIE) ? rcut / (rcut - rc) : 1.0;
             h.ampg = b0 * t05 * ci;
             h.amph = b0 * t05 * ci;

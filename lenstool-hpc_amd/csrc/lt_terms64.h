@@ -158,54 +158,54 @@ __device__ __forceinline__ void hess_sis_circ(const HaloRec& h, double px, doubl
 
 // N conj(D) of the I_{0.5} ratio Z = N/D (N = cx1 x + i t, D = x + i n2); its real part is
 // non-negative everywhere, so arg Z lies in [-pi/2, pi/2].
-struct NconjD {
-    double re, im, den1, den2;
+// Z = N / D of the reference's I_{0.5} ratio (src/gradient.hpp:64-76), N = cx1 x + i (2 s w - y / cx1),
+// D = x + i (2 r s - y) with s = sqrt(e), w = sqrt(r^2 + a^2 x^2 + b^2 y^2), a = 1/(1+e), b = 1/(1-e),
+// taken in the form of the Hessian's division-free derivation (lt_kernels32.hip):
+//   N / D = Q / S,   Q = r + cx1 w + i 2 s a^2 x,   S = w + r cx1 + 2 s a b y > 0,
+// so arg Z = arg Q (Re Q >= r > 0) and ln|Z| = ln(|Q|^2 / S^2) / 2. N and D both vanish at the
+// halo-frame point (0, 2 r s) and are differences of nearly equal numbers next to it, where the closed
+// form loses up to 1e-7 relative; Q and S have no such cancellation.
+struct QS64 {
+    double R, Q2, S;  // Re Q, |Q|^2, S
 };
 
-__device__ __forceinline__ NconjD piemd_ratio(const HaloRec& h, const PixelFrame& f, double rem2, double yi,
-                                              double cxx, double r2, double e2r)
+__device__ __forceinline__ QS64 qs64(const HaloRec& h, double rem2, double I2, double vy, double r, double r2,
+                                     double rcx)
 {
-    double q = r2 + rem2;
-    double w = q * rsq64(q);
-    double t = __builtin_fma(h.e2, w, -yi);
-    double n2 = e2r - f.y;
-    NconjD o;
-    o.re = __builtin_fma(t, n2, h.cx1 * f.x2);
-    o.im = f.x * __builtin_fma(-h.cx1, n2, t);
-    o.den1 = __builtin_fma(t, t, cxx);
-    o.den2 = __builtin_fma(n2, n2, f.x2);
+    const double q = r2 + rem2;
+    const double w = q * rsq64(q);
+    QS64 o;
+    o.R = __builtin_fma(h.cx1, w, r);
+    o.Q2 = __builtin_fma(o.R, o.R, I2);
+    o.S = (w + rcx) + vy;
     return o;
 }
 
-// PIEMD (CUT = false) or dPIE (CUT = true): (ampg arg, -ampg ln|.|) of Z(rc) or Z(rc)/Z(rcut).
-// For the dPIE the two logarithms and the two arguments are merged into one of each:
-// ln|Za| - ln|Zb| = ln(den1a den2b / (den2a den1b)) / 2, arg Za - arg Zb = arg(Na conj(Da) Nb' Db)
-// (both arguments in [-pi/2, pi/2], so the difference needs no unwrapping).
+// PIEMD (CUT = false) or dPIE (CUT = true): (ampg arg, -ampg ln|.|) of Z(rc) or Z(rc) / Z(rcut). For
+// the dPIE the two logarithms and the two arguments are merged into one of each, with I = Im Q the same
+// for both radii: arg(Za / Zb) = atan2(I (Rb - Ra), Ra Rb + I^2) (Ra Rb + I^2 > 0: no unwrapping) and
+// ln|Za / Zb| = ln(|Qa|^2 Sb^2 / (|Qb|^2 Sa^2)) / 2.
 template <bool CUT>
 __device__ __forceinline__ void grad_piemd(const HaloRec& h, double px, double py, double& gx, double& gy)
 {
     PixelFrame f = halo_frame(h, px, py);
-    double rem2 = __builtin_fma(f.x2, h.kx, f.y2 * h.ky);
-    double yi = f.y * h.icx1;
-    double cxx = h.cx1sq * f.x2;
-    NconjD a = piemd_ratio(h, f, rem2, yi, cxx, h.ra2, h.e2ra);
-    double pre, pim, num, den;
+    const double rem2 = __builtin_fma(f.x2, h.kx, f.y2 * h.ky);
+    const double I = h.gki * f.x;
+    const double I2 = I * I;
+    const double vy = h.gkv * f.y;
+    const QS64 a = qs64(h, rem2, I2, vy, h.rc, h.ra2, h.rca);
+    double arg, ratio;
     if (CUT) {
-        NconjD b = piemd_ratio(h, f, rem2, yi, cxx, h.rb2, h.e2rb);
-        pre = __builtin_fma(a.re, b.re, a.im * b.im);
-        pim = __builtin_fma(a.im, b.re, -(a.re * b.im));
-        num = a.den1 * b.den2;
-        den = a.den2 * b.den1;
+        const QS64 b = qs64(h, rem2, I2, vy, h.rcut, h.rb2, h.rcb);
+        arg = atan2_lt(I * (b.R - a.R), __builtin_fma(a.R, b.R, I2));
+        ratio = (a.Q2 * (b.S * b.S)) * rcp64(b.Q2 * (a.S * a.S));
     } else {
-        pre = a.re;
-        pim = a.im;
-        num = a.den1;
-        den = a.den2;
+        arg = atan2_lt(I, a.R);
+        ratio = a.Q2 * rcp64(a.S * a.S);
     }
-    double arg = atan2_lt(pim, pre);
-    double lnr = 0.5 * log_lt(num * rcp64(den));
-    double re = h.ampg * arg;
-    double im = -h.ampg * lnr;
+    const double lnr = 0.5 * log_lt(ratio);
+    const double re = h.ampg * arg;
+    const double im = -h.ampg * lnr;
     gx += __builtin_fma(re, h.cs, -(im * h.sn));
     gy += __builtin_fma(im, h.cs, re * h.sn);
 }

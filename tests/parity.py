@@ -9,8 +9,8 @@ its first-order FP64 rounding estimate `cond` (orc_cond_planes) is large; in 80-
 error shrinks by the ratio of the unit roundoffs, 2^-64 / 2^-53 = 2^-11.
   * The GPU Hessian uses a form without that cancellation, so against the extended oracle it gets
     E = 2^-10 cond (the oracle's own share, with a factor 2 to spare): EXT_SHARE.
-  * The GPU FP64 gradient evaluates the reference's closed form, as the reference does, and keeps
-    E = cond: its rounding next to the removable point is the reference's own.
+  * The GPU FP64 gradient takes the I_{0.5} ratio in a form without that cancellation as well
+    (Q / S, lt_terms64.h grad_piemd): E = EXT_SHARE too.
   * Comparisons with numbers of the reference's FP64 build (golden grids) keep E = cond.
 """
 import numpy as np

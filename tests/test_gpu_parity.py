@@ -66,8 +66,8 @@ def test_gradient_mixed(hip, oracle, istart, jstart):
     ex, ey = oracle.gradient_grid(lens, grid, istart, jstart, extended=True)
     _, _, ax, ay = oracle.gradient_grid(lens, grid, istart, jstart, with_abs=True)
     cg, _ = oracle.cond_planes(lens, grid, istart, jstart)
-    assert_close("gx", gx, ex, ax, cg)
-    assert_close("gy", gy, ey, ay, cg)
+    assert_close("gx", gx, ex, ax, EXT_SHARE * cg)
+    assert_close("gy", gy, ey, ay, EXT_SHARE * cg)
 
 
 def test_gradient_group_lens(hip, oracle):
@@ -80,9 +80,9 @@ def test_gradient_group_lens(hip, oracle):
     ex, ey = oracle.gradient_grid(lens, grid, extended=True)
     _, _, ax, ay = oracle.gradient_grid(lens, grid, with_abs=True)
     cg, _ = oracle.cond_planes(lens, grid)
-    assert_close("gx", gx, ex, ax, cg)
-    assert_close("gy", gy, ey, ay, cg)
-    # against the double oracle (the reference's precision) as well
+    assert_close("gx", gx, ex, ax, EXT_SHARE * cg)
+    assert_close("gy", gy, ey, ay, EXT_SHARE * cg)
+    # against the double oracle (the reference's precision: its own rounding, cond) as well
     dx, dy = oracle.gradient_grid(lens, grid)
     assert_close("gx-double", gx, dx, ax, 2 * cg)
 
@@ -383,7 +383,7 @@ def test_real_model_a2744(hip, oracle):
     ex, ey = oracle.gradient_grid(lens, grid, extended=True)
     _, _, ax, ay = oracle.gradient_grid(lens, grid, with_abs=True)
     cg, _ = oracle.cond_planes(lens, grid)
-    assert (np.abs(gx - ex) <= REL * ax + cg).all() and (np.abs(gy - ey) <= REL * ay + cg).all()
+    assert (np.abs(gx - ex) <= REL * ax + EXT_SHARE * cg).all() and (np.abs(gy - ey) <= REL * ay + EXT_SHARE * cg).all()
     r = hip.cosmo_dlsds(0.308, 2.0, d["cosmo"])
     f64 = hip.map_fields(lens, grid, d["cosmo"], 0.308, 2.0)
     f32 = hip.map_fields(lens, grid, d["cosmo"], 0.308, 2.0, precision=hip.PREC_FP32C)
