@@ -21,7 +21,8 @@ struct lt_chi_batch_s {
     lt_grid_param frame{};
     double dx = 0, dy = 0;
     std::vector<int> types;
-    std::vector<HaloRec> staging;
+    HaloRec* staging = nullptr;  // pinned host image of the samples' records (one async upload per run)
+    hipEvent_t staged = nullptr;  // recorded after that upload: the next run refills staging after it
     std::vector<Run> runs;
     void* block = nullptr;
     HaloRec* recs = nullptr;
@@ -200,7 +201,15 @@ int lt_chi_batch_create(lt_chi_batch* out, int nsamples, int nhalos, const int* 
         delete b;
         return set_error(LT_ERR_HIP, "chi batch: run upload failed");
     }
-    b->staging.resize((size_t)nsamples * std::max(nhalos, 1));
+    if (hipHostMalloc((void**)&b->staging, (size_t)nsamples * std::max(nhalos, 1) * sizeof(HaloRec),
+                      hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&b->staged, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        if (b->staging) (void)hipHostFree(b->staging);
+        (void)hipFree(b->block);
+        delete b;
+        return set_error(LT_ERR_NOMEM, "chi batch: pinned staging allocation failed");
+    }
     *out = b;
     return LT_OK;
 }
@@ -210,6 +219,12 @@ int lt_chi_batch_destroy(lt_chi_batch b)
     if (!b) return LT_OK;
     (void)hipSetDevice(b->device);
     hipError_t e = b->block ? hipFree(b->block) : hipSuccess;
+    if (b->staged) {
+        (void)hipEventSynchronize(b->staged);
+        (void)hipEventDestroy(b->staged);
+    }
+    const hipError_t eh = b->staging ? hipHostFree(b->staging) : hipSuccess;
+    if (e == hipSuccess) e = eh;
     delete b;
     if (e != hipSuccess) return hip_error(e, "chi batch free");
     return LT_OK;
@@ -225,6 +240,8 @@ int lt_chi_batch_run(lt_chi_batch b, const lt_potential_soa* lenses, double* chi
     std::vector<HaloRec32> recs32;
     std::vector<Run> runs;
     std::string err;
+    // the previous run's upload may still read the staging area if that run stopped on an error
+    LT_HIP_TRY(hipEventSynchronize(b->staged));
     // a run uses the circular-SIS terms only when it is circular in every sample
     for (Run& r : b->runs) r.circ = 1;
     for (int i = 0; i < b->S; ++i) {
@@ -239,8 +256,9 @@ int lt_chi_batch_run(lt_chi_batch b, const lt_potential_soa* lenses, double* chi
         if (b->H) std::memcpy(&b->staging[(size_t)i * b->H], recs.data(), (size_t)b->H * sizeof(HaloRec));
     }
     if (b->H)
-        LT_HIP_TRY(hipMemcpyAsync(b->recs, b->staging.data(), (size_t)b->S * b->H * sizeof(HaloRec),
+        LT_HIP_TRY(hipMemcpyAsync(b->recs, b->staging, (size_t)b->S * b->H * sizeof(HaloRec),
                                   hipMemcpyHostToDevice, s));
+    LT_HIP_TRY(hipEventRecord(b->staged, s));
     if (!b->runs.empty())
         LT_HIP_TRY(hipMemcpyAsync(b->druns, b->runs.data(), b->runs.size() * sizeof(Run), hipMemcpyHostToDevice, s));
     int st = reset_counters(b, s);
