@@ -111,12 +111,22 @@ constexpr int SCAN_UNROLL = 4;   // sources culled per step of the scan loop
 // few cells around each image, degenerate or non-finite cells), one source at a time. RHO_ONLY: every
 // lane's cell has d_ok (or is past the row end), so the margin needs only the distance.
 template <bool RHO_ONLY>
-__device__ __forceinline__ void scan_pass(const DelenseDesc& d, const CellCorners& c, const CellFilter& f, bool active,
-                                          int s, int k0, int cell, int cnt, const double* lsx, const double* lsy,
-                                          const double* ldr)
+__device__ __forceinline__ void scan_pass(const DelenseDesc& d, const CellCorners& c, const CellFilter& f,
+                                          const CellFilter32& f32, bool active, int s, int k0, int cell, int cnt,
+                                          const double* lsx, const double* lsy, const double* ldr, const float* lsx32,
+                                          const float* lsy32, const float* ldr32)
 {
     int j0 = 0;
     for (; j0 + SCAN_UNROLL <= cnt; j0 += SCAN_UNROLL) {
+        if constexpr (RHO_ONLY) {
+            // FP32 pre-test of the step (lt_delense_geom.h cell_filter32): sound on its own, and it decides
+            // nearly every step; the FP64 margin below runs only where it does not
+            float mg32 = far_margin_rho32(f32, lsx32[j0], lsy32[j0], ldr32[j0]);
+#pragma unroll
+            for (int u = 1; u < SCAN_UNROLL; ++u)
+                mg32 = __builtin_fminf(mg32, far_margin_rho32(f32, lsx32[j0 + u], lsy32[j0 + u], ldr32[j0 + u]));
+            if (!active | (mg32 > 0.f)) continue;
+        }
         double mg = RHO_ONLY ? far_margin_rho(f, c, lsx[j0], lsy[j0], ldr[j0]) : far_margin(f, c, lsx[j0], lsy[j0], ldr[j0]);
 #pragma unroll
         for (int u = 1; u < SCAN_UNROLL; ++u)
@@ -137,6 +147,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_delense(const double* __restrict
                                                         DelenseDesc d)
 {
     __shared__ double lsx[SCAN_BLOCK], lsy[SCAN_BLOCK], ldr[SCAN_BLOCK];
+    __shared__ float lsx32[SCAN_BLOCK], lsy32[SCAN_BLOCK], ldr32[SCAN_BLOCK];
     const int n = d.n;
     const int xid = blockIdx.x * SCAN_BLOCK + threadIdx.x;
     const int yid = blockIdx.y;  // the grid has n - 1 rows of blocks
@@ -148,19 +159,24 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_delense(const double* __restrict
     const CellCorners c = load_corners(gx + base, gy + base, d, active ? xid : n - 2, yid);
     const double* sb = d.src_bound + 3 * (int64_t)s;
     const CellFilter f = cell_filter(c, sb[0], sb[1], sb[2]);
+    const CellFilter32 f32 = cell_filter32(f, c);
     const bool rho_only = __all(!active || f.d_ok);  // uniform over the wavefront
     for (int k0 = 0; k0 < d.nsets; k0 += SCAN_BLOCK) {
         const int cnt = min(SCAN_BLOCK, d.nsets - k0);
         __syncthreads();  // the previous pass is done with the staged sources
         if ((int)threadIdx.x < cnt) {
             const int64_t sk = (int64_t)s * d.nsets + k0 + threadIdx.x;
-            lsx[threadIdx.x] = d.src_x[sk];
-            lsy[threadIdx.x] = d.src_y[sk];
-            ldr[threadIdx.x] = d.src_dr[sk];
+            const double sx = d.src_x[sk], sy = d.src_y[sk], sdr = d.src_dr[sk];
+            lsx[threadIdx.x] = sx;
+            lsy[threadIdx.x] = sy;
+            ldr[threadIdx.x] = sdr;
+            lsx32[threadIdx.x] = (float)sx;
+            lsy32[threadIdx.x] = (float)sy;
+            ldr32[threadIdx.x] = (float)sdr;
         }
         __syncthreads();
-        if (rho_only) scan_pass<true>(d, c, f, active, s, k0, cell, cnt, lsx, lsy, ldr);
-        else scan_pass<false>(d, c, f, active, s, k0, cell, cnt, lsx, lsy, ldr);
+        if (rho_only) scan_pass<true>(d, c, f, f32, active, s, k0, cell, cnt, lsx, lsy, ldr, lsx32, lsy32, ldr32);
+        else scan_pass<false>(d, c, f, f32, active, s, k0, cell, cnt, lsx, lsy, ldr, lsx32, lsy32, ldr32);
     }
 }
 

@@ -97,6 +97,7 @@ __host__ __device__ inline int cell_hit(const CellCorners& c, double px, double 
 // an end.
 struct CellFilter {
     double c0, c1u, c2u, c1l, c2l, rn2, tau;
+    double m;  // the coordinate bound M
     bool on;
     bool d_ok;  // |d| > tau for every dr of the range (both triangles): the margin needs only rho
 };
@@ -134,6 +135,7 @@ __host__ __device__ inline CellFilter cell_filter(const CellCorners& c, double p
     const double mc0 = max_d(max_d(abs_d(c.xp), abs_d(c.yp)), max_d(abs_d(c.xq), abs_d(c.yq)));
     const double drmax = max_d(abs_d(drlo), abs_d(drhi));
     const double m = max_d(mc0 + drmax * mc1, pmax) * 1.000001;
+    f.m = m;
     f.on = sg <= 1e100 && sp <= 1e100 && pmax <= 1e100 && drmax <= 1e100 && drlo <= drhi && m >= 1e-50 &&
            m <= 1e100;
     f.tau = 1e-10 * m * m;
@@ -183,6 +185,39 @@ __host__ __device__ inline double far_margin_rho(const CellFilter& f, const Cell
     const double ex = px - __builtin_fma(-dr, c.g0x, c.xp);
     const double ey = py - __builtin_fma(-dr, c.g0y, c.yp);
     return __builtin_fma(ex, ex, ey * ey) - f.rn2;
+}
+
+// FP32 pre-test of far_margin_rho for cells with d_ok. Every input rounded to FP32 (relative error
+// u = 2^-24 of at most M: the coordinates, the sources and dr |g| are bounded by M), then
+// ax = px - (xp - dr g0x) formed with one FMA and one subtraction: the FP32 offset differs from the
+// FP64 one by at most 9 u M per axis (input roundings u M for xp, 2 u M for dr g0x, u M for px, two
+// operation roundings within 2 u M and 3 u M), so rho differs by at most 13 u M < 7.8e-7 M. The squared
+// radius rf2 is that of sqrt(rn2) + 1e-6 M, enlarged by a further 1e-6 relative for the rounding of
+// rho^2 and of rf2 itself: a positive FP32 margin implies rho^2 > rn2 in exact arithmetic, which is
+// what the FP64 margin establishes. Off (rf2 = +infinity) unless d_ok and M <= 1e30.
+struct CellFilter32 {
+    float xp, yp, g0x, g0y, rf2;
+};
+
+__host__ __device__ inline CellFilter32 cell_filter32(const CellFilter& f, const CellCorners& c)
+{
+    CellFilter32 g;
+    g.xp = (float)c.xp;
+    g.yp = (float)c.yp;
+    g.g0x = (float)c.g0x;
+    g.g0y = (float)c.g0y;
+    const bool ok = f.d_ok && f.m <= 1e30;
+    const double rf = __builtin_sqrt(f.rn2) + 1e-6 * f.m;
+    g.rf2 = ok ? (float)(rf * rf * (1. + 1e-6)) : __builtin_inff();
+    return g;
+}
+
+__host__ __device__ inline float far_margin_rho32(const CellFilter32& g, float px, float py, float dr)
+{
+#pragma clang fp contract(off)
+    const float ex = px - __builtin_fmaf(-dr, g.g0x, g.xp);
+    const float ey = py - __builtin_fmaf(-dr, g.g0y, g.yp);
+    return __builtin_fmaf(ex, ex, ey * ey) - g.rf2;
 }
 
 // true: both of cell_hit()'s triangle tests provably fail for source (px, py, dr), by the near or the

@@ -7,7 +7,11 @@
 //      |tri_det computed - exact| <= 4.0e-15 M^2 and |d(dr) polynomial - exact det of the computed
 //      corners| <= 3.2e-14 M^2 (M = largest coordinate involved);
 //   3. efficiency: on lens-like cells (0.05" or larger) far from the source the fast path's margin
-//      decides almost every pair.
+//      decides almost every pair;
+//   4. the FP32 pre-test (d_ok && far_margin_rho32() > 0) is sound on its own: whenever it skips,
+//      cell_hit() finds no hit and the exact distance from the source to the cell's corner g exceeds
+//      sqrt(rn2), which is what the FP64 margin establishes; on lens-like far pairs it decides about as
+//      often as the FP64 margin.
 // Cells come from smooth lens-like gradient fields, near-collinear and collapsed mapped triangles,
 // wild and huge gradients, sources on corners and edges, and NaN / infinity in gradients, sources and
 // D_ls/D_s. Exit status 0 = all checks hold; the counts are printed.
@@ -34,7 +38,7 @@ static double amax(double a, double b) { return std::fabs(a) > std::fabs(b) ? st
 
 struct Stats {
     long trials = 0, culled = 0, hits = 0, unsound = 0, det_bad = 0, poly_bad = 0, far_pairs = 0, far_culled = 0,
-         dok = 0, dok_bad = 0;
+         dok = 0, dok_bad = 0, fast32 = 0, fp32_bad = 0, far_rho = 0, far_culled32 = 0;
     double det_worst = 0, poly_worst = 0;
 };
 
@@ -45,12 +49,20 @@ static void check(Stats& st, const CellCorners& c, double px, double py, double 
     const bool far = lt::far_from_cell(f, c, px, py, dr);
     const bool fast = (f.on && lt::far_margin(f, c, px, py, dr) > 0.) ||
                       (f.d_ok && lt::far_margin_rho(f, c, px, py, dr) > 0.);
+    const lt::CellFilter32 f32 = lt::cell_filter32(f, c);
+    const bool fast32 = f.d_ok && lt::far_margin_rho32(f32, (float)px, (float)py, (float)dr) > 0.f;
+    st.fast32 += fast32;
+    if (fast32) {
+        // the exact distance from the source to the exact corner g = (xp - dr g0x, yp - dr g0y)
+        const q ex = (q)px - ((q)c.xp - (q)dr * c.g0x), ey = (q)py - ((q)c.yp - (q)dr * c.g0y);
+        if (!(ex * ex + ey * ey > (q)f.rn2)) ++st.fp32_bad;
+    }
     const int hit = lt::cell_hit(c, px, py, dr);
     ++st.trials;
     st.culled += far;
     st.dok += f.d_ok;
     st.hits += hit != 0;
-    if ((far || fast) && hit) {
+    if ((far || fast || fast32) && hit) {
         if (st.unsound < 5)
             std::printf("UNSOUND: corners (%.17g %.17g %.17g %.17g) g (%.17g %.17g | %.17g %.17g | %.17g %.17g | %.17g "
                         "%.17g) P (%.17g %.17g) dr %.17g\n",
@@ -60,6 +72,11 @@ static void check(Stats& st, const CellCorners& c, double px, double py, double 
     if (lens_like_far) {
         ++st.far_pairs;
         st.far_culled += fast;
+        // the pairs the kernel's FP32 pre-test is for: d_ok cells whose FP64 distance margin decides
+        if (f.d_ok && lt::far_margin_rho(f, c, px, py, dr) > 0.) {
+            ++st.far_rho;
+            st.far_culled32 += fast32;
+        }
     }
     if (!f.on || !std::isfinite(px) || !std::isfinite(py) || !std::isfinite(dr)) return;
     // the error bounds behind the proof, on the corners the reference computes
@@ -229,11 +246,15 @@ int main(int argc, char** argv)
     std::printf("tri_det error worst %.3g M^2 (bound 4.0e-15, over: %ld); d(dr) error worst %.3g M^2 (bound 3.2e-14, over: %ld)\n",
                 st.det_worst, st.det_bad, st.poly_worst, st.poly_bad);
     std::printf("cells with d_ok %ld (violations %ld)\n", st.dok, st.dok_bad);
-    std::printf("lens-like far pairs %ld culled %ld (%.4f)\n", st.far_pairs, st.far_culled,
-                st.far_pairs ? (double)st.far_culled / st.far_pairs : 0.0);
+    std::printf("lens-like far pairs %ld culled %ld (%.4f); of the %ld with d_ok decided by the FP64 distance "
+                "margin, the FP32 pre-test decides %ld (%.4f)\n", st.far_pairs, st.far_culled,
+                st.far_pairs ? (double)st.far_culled / st.far_pairs : 0.0, st.far_rho, st.far_culled32,
+                st.far_rho ? (double)st.far_culled32 / st.far_rho : 0.0);
+    std::printf("FP32 pre-test skips %ld (exact distance not above sqrt(rn2): %ld)\n", st.fast32, st.fp32_bad);
     const bool ok = st.unsound == 0 && st.det_bad == 0 && st.poly_bad == 0 && st.dok_bad == 0 && st.dok > 0 &&
-                    st.hits > 0 &&
-                    st.far_pairs > 0 && st.far_culled >= 0.999 * st.far_pairs;
+                    st.hits > 0 && st.fp32_bad == 0 && st.fast32 > 0 &&
+                    st.far_pairs > 0 && st.far_culled >= 0.999 * st.far_pairs &&
+                    st.far_rho > 0 && st.far_culled32 >= 0.99 * st.far_rho;
     std::printf("%s\n", ok ? "OK" : "FAIL");
     return ok ? 0 : 1;
 }
