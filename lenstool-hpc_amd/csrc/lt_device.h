@@ -13,10 +13,10 @@ namespace lt {
 
 // One halo, 48 doubles. Field meaning by type:
 //   all     : xh, yh (centre), cs, sn (cos/sin of the angle), hc2, hs2 (Hessian amplitude times
-//             cos/sin of twice the angle), b0, eps, rc, rcut (raw parameters, for the potential)
+//             cos/sin of twice the angle)
 //   8, 81   : PIEMD constants (see below); ampg / amph gradient / Hessian amplitudes; csx .. k5,
-//             ra2c, rb2c, qT .. qs2h the constants of the Hessian's scaled division-free form
-//             (lt_terms64.h)
+//             ra2c, rb2c, qT .. qs2h the constants of the Hessian's scaled division-free form;
+//             psra .. picra those of the potential (lt_terms64.h)
 //   5       : sa = 1 - eps, sb = 1 + eps, ampg = b0, amph = b0 (1 - eps)(1 + eps); circ = 1 when
 //             eps = 0 (runs of circular SIS form their terms from the lab-frame offsets)
 //   14      : h14a, h14b, h14c = constant Hessian (lab frame)
@@ -32,16 +32,25 @@ struct alignas(64) HaloRec {
     double amph;                    // 17     Hessian amplitude
     double csx, snx, csy, sny;      // 18..21 cx1 a (cs, sn), cx1 b (cs, sn): the scaled Hessian frame
     double ra2c, rb2c;              // 22, 23 cx1^2 rcore^2, cx1^2 rcut^2
-    double sa, sb;                  // 24, 25 SIS (1 - e), (1 + e)
-    double h14a, h14b, h14c;        // 26..28 type-14 constant Hessian
-    double b0, eps, rc, rcut;       // 29..32 raw parameters (potential)
+    union {                         // 24..28 by type:
+        struct {
+            double sa, sb;          //   SIS (1 - e), (1 + e)
+            double h14a, h14b, h14c;  // type-14 constant Hessian
+        };
+        struct {                    //   PIEMD potential, dPIE merging of the two radii (lt_terms64.h):
+            double pc, pomc;        //   c = 1 + cx1, 1 - cx1
+            double pdr, pcdr, pomcdr;  // D = rcut - rcore, c D, (1 - cx1) D
+        };
+    };
+    double psra, psrb;              // 29, 30 potential: sqrt(e) rcore, sqrt(e) rcut
+    double rc, rcut;                // 31, 32 rcore, rcut
     double k1, k2, k3;              // 33..35 2 sqrt(e) a / cx1, b / a, 2 sqrt(e) b / cx1
     double qT, qc2, qs2m2, qs2h;    // 36..39 Hessian amplitudes of the division-free form
     double k4, k5;                  // 40, 41 1 / cx1^2, 2 sqrt(e) a   (a, b = 1/(1 +- e))
-    double pkp, pkq;                // 42, 43 potential: 2 sqrt(e) / (1 - e), 2 sqrt(e) / (1 + e)
-    double irc, ircut;              // 44, 45 1 / rcore, 1 / rcut
+    double pcra, pcrb;              // 42, 43 potential: c rcore, c rcut (c = 1 + cx1 = 2 / (1 + e))
+    double picra2, picrb2;          // 44, 45 1 / (c rcore)^2, 1 / (c rcut)^2
     double circ;                    // 46     1 for a circular SIS (e = 0)
-    double pad;                     // 47
+    double picra;                   // 47     1 / (c rcore)
 };
 static_assert(sizeof(HaloRec) == 384, "halo record is 384 bytes");
 
@@ -94,17 +103,16 @@ struct alignas(64) GradRec32 {
 static_assert(sizeof(GradRec32) == sizeof(HaloRec32), "gradient fp32 record is 128 bytes");
 
 // Single-precision record of a halo for the FP32-compensated potential (models packed for potential
-// use hold these in place of HaloRec32, same size): the constants of the algebraic I_{0.5} potential
-// of lt_terms64.h pot_halo in FP32.
+// use hold these in place of HaloRec32, same size): the constants of lt_terms64.h pot_piemd in FP32.
 struct alignas(64) PotRec32 {
     float cs, sn;                       // 0, 1    cos, sin of the angle
-    float kx, ky;                       // 2, 3    1/(1+e)^2, 1/(1-e)^2
-    float pkp, pkq;                     // 4, 5    2 sqrt(e)/(1-e), 2 sqrt(e)/(1+e)
-    float ra, ira, ra2;                 // 6..8    rcore, 1/rcore, rcore^2
-    float rb, irb, rb2;                 // 9..11   rcut, 1/rcut, rcut^2
-    float amp;                          // 12      PIEMD / dPIE amplitude (ampg); SIS b0
-    float sa, sb;                       // 13, 14  SIS: 1 - e, 1 + e
-    float pad[17];                      // 15..31
+    float kx, ky, cx1, gki, gkv;        // 2..6    as HaloRec
+    float ra, ra2, cra, icra2, icra, sra;  // 7..12  rcore, rcore^2, c rcore, 1/(c rcore)^2, 1/(c rcore), sqrt(e) rcore
+    float rb, rb2, crb, icrb2, srb;     // 13..17  the same for rcut (c = 1 + cx1)
+    float pc, pomc, pdr, pcdr, pomcdr;  // 18..22  c, 1 - cx1, D = rcut - rcore, c D, (1 - cx1) D
+    float amp;                          // 23      PIEMD / dPIE amplitude (ampg); SIS b0
+    float sa, sb;                       // 24, 25  SIS: 1 - e, 1 + e
+    float pad[6];                       // 26..31
 };
 static_assert(sizeof(PotRec32) == sizeof(HaloRec32), "potential fp32 record is 128 bytes");
 

@@ -874,20 +874,16 @@ hipError_t launch_gradient32(const HaloRec* h64, const HaloRec32* recs32, const 
 }
 
 // ------------------------------------------------------------------------------------------------
-// FP32-compensated potential: the algebraic I_{0.5} potential of lt_terms64.h (pot_halo: both
-// logarithms as log1p of quantities that vanish at the centre, the second ones of the two radii
-// merged into one, instead of the reference's complex cosh / sinh / log,
-// src/potential_GPU.cu:47-116) evaluated in FP32 with the single-precision arctangent, a series
-// log1p and the hardware base-2 logarithm; the offset pixel - centre formed in FP64 and rounded once; FP32 terms
-// summed per chunk of CHUNK halos into FP64 run sums. Every term is FP32 (the potential has no
-// singular core: it is finite and smooth at the centre of a cored halo), and the map reports its
-// error against the FP64 kernel. Floating-point contraction is off in these functions: only the
-// written FMAs are fused, so that every launch shape (pixels per thread) evaluates the same operations
-// and gives the same map.
+// FP32-compensated potential: the form of lt_terms64.h pot_piemd (phi_r = x arg Q + y alpha_y -
+// 2 sqrt(e) r ln((1 + e) |Q| / (2 r)) with log1p logarithms and, for the dPIE, the deflection parts
+// of the two radii merged without cancellation; the reference's is src/potential_GPU.cu:47-116)
+// evaluated in FP32 with the single-precision arctangent, a series log1p and the hardware base-2
+// logarithm; the offset pixel - centre formed in FP64 and rounded once; FP32 terms summed per chunk
+// of CHUNK halos into FP64 run sums. Every term is FP32 (the potential has no singular core: it is
+// finite and smooth at the centre of a cored halo), and the map reports its error against the FP64
+// kernel. Floating-point contraction is off in these functions: only the written FMAs are fused, so
+// that every launch shape (pixels per thread) evaluates the same operations and gives the same map.
 // ------------------------------------------------------------------------------------------------
-struct PotShared32 {
-    float p, q, PC, pq, Nr, Ni, rem2, isrem, srem, onePC, kr, ki;
-};
 
 // ln(1 + a) in FP32 for a > -1: 2 atanh(s), s = a / (2 + a), by its series through s^9 for
 // -0.3 < a < 0.4 (|s| < 0.18: the next term is below 4e-9 relative), the hardware logarithm of 1 + a
@@ -905,28 +901,37 @@ __device__ __forceinline__ float log1pf_lt(float a)
     return (a > -0.3f && a < 0.4f) ? series : logf_lt(1.f + a);
 }
 
-// the core-radius part: a1 = -ln(1 + z), z = (rem / rc) t / D, t = rem / (rc + W) (lt_terms64.h
-// pot_core)
-struct PotCore32 {
-    float a1r, a1i, t;
-};
-
-__device__ __forceinline__ PotCore32 pot_core32(const PotShared32& s, float rc, float irc, float rc2)
+// min(a, b) of two non-NaN floats as one instruction
+__device__ __forceinline__ float min32(float a, float b)
 {
-#pragma clang fp contract(off)
-    const float W2 = rc2 + s.rem2;
-    const float W = W2 * rsq32(W2);
-    PotCore32 c;
-    c.t = s.srem * rcp32(rc + W);
-    const float w1 = (s.srem * irc) * c.t;
-    const float zr = w1 * s.kr, zi = w1 * s.ki;
-    c.a1r = -0.5f * log1pf_lt(__builtin_fmaf(zi, zi, zr * (2.f + zr)));
-    c.a1i = -atan2f_lt(zi, 1.f + zr);
-    return c;
+    float r;
+    asm("v_min_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
 }
 
+// one core radius: w, d = w - r, Re Q = c r + cx1 d, lq = ln(|Q|^2 / (c r)^2) (lt_terms64.h pot_rad)
+struct PotRad32 {
+    float w, d, R, lq;
+};
+
+__device__ __forceinline__ PotRad32 pot_rad32(float cx1, float rem2, float I2, float r, float r2, float cr,
+                                              float icr2)
+{
+#pragma clang fp contract(off)
+    const float q = r2 + rem2;
+    PotRad32 o;
+    o.w = q * rsq32(q);
+    o.d = rem2 * rcp32(o.w + r);
+    const float cd = cx1 * o.d;
+    o.R = cd + cr;
+    o.lq = log1pf_lt(__builtin_fmaf(cd, __builtin_fmaf(2.f, cr, cd), I2) * icr2);
+    return o;
+}
+
+// one halo's potential at the FP32 offset (tx, ty); mn = min(mn, rem^2) over PIEMD halos (a pixel on a
+// centre, where the reference's potential is 0/0, gets NaN)
 template <int TYPE>
-__device__ __forceinline__ float pot32(const PotRec32& h, float tx, float ty)
+__device__ __forceinline__ float pot32(const PotRec32& h, float tx, float ty, float& mn)
 {
 #pragma clang fp contract(off)
     const float x = __builtin_fmaf(tx, h.cs, ty * h.sn);
@@ -936,61 +941,38 @@ __device__ __forceinline__ float pot32(const PotRec32& h, float tx, float ty)
         const float R2 = __builtin_fmaf(h.sa, x2, h.sb * y2);
         return h.amp * (R2 * rsq32(R2));
     } else {
-        PotShared32 s;
-        const float iz = rsq32(x2 + y2);
-        s.p = h.pkp * y * iz;
-        s.q = h.pkq * x * iz;
-        const float P2 = __builtin_fmaf(s.p, s.p, 1.f), C2 = __builtin_fmaf(-s.q, s.q, 1.f);
-        const float P = P2 * rsq32(P2), C = C2 * rsq32(C2);
-        s.PC = P * C;
-        s.pq = s.p * s.q;
-        s.Nr = -s.p * C;
-        s.Ni = P * s.q;
-        s.rem2 = __builtin_fmaf(x2, h.kx, y2 * h.ky);
-        s.isrem = rsq32(s.rem2);
-        s.srem = s.rem2 * s.isrem;
-        s.onePC = 1.f + s.PC;
-        const float iD2 = rcp32(__builtin_fmaf(s.onePC, s.onePC, s.pq * s.pq));
-        s.kr = s.onePC * iD2;
-        s.ki = s.pq * iD2;
-        const PotCore32 a = pot_core32(s, h.ra, h.ira, h.ra2);
-        // a2 = ln(1 + z2), z2 = U / V (lt_terms64.h pot_halo)
-        const float amr = __builtin_fmaf(-a.t, s.Nr, s.onePC), ami = __builtin_fmaf(-a.t, s.Ni, -s.pq);
-        float cr, ci, Ur, Ui, Vr, Vi;
+        const float rem2 = __builtin_fmaf(x2, h.kx, y2 * h.ky);
+        mn = min32(mn, rem2);
+        const float I = h.gki * x;
+        const float I2 = I * I;
+        const float v = h.gkv * y;
+        const PotRad32 a = pot_rad32(h.cx1, rem2, I2, h.ra, h.ra2, h.cra, h.icra2);
         if constexpr (TYPE == TYPE_PIEMD) {
-            Ur = 2.f * a.t * s.Nr;
-            Ui = 2.f * a.t * s.Ni;
-            Vr = amr;
-            Vi = ami;
-            cr = h.ra * a.a1r;
-            ci = h.ra * a.a1i;
+            const float ls = log1pf_lt((a.d + v) * h.icra);
+            const float arg = atan2f_lt(I, a.R);
+            const float ay = __builtin_fmaf(-0.5f, a.lq, ls);
+            return h.amp * __builtin_fmaf(x, arg, __builtin_fmaf(y, ay, -(h.sra * a.lq)));
         } else {
-            const PotCore32 b = pot_core32(s, h.rb, h.irb, h.rb2);
-            const float bpr = __builtin_fmaf(b.t, s.Nr, s.onePC), bpi = __builtin_fmaf(b.t, s.Ni, -s.pq);
-            Vr = __builtin_fmaf(amr, bpr, -(ami * bpi));
-            Vi = __builtin_fmaf(amr, bpi, ami * bpr);
-            const float dt = 2.f * (a.t - b.t);
-            const float dnr = __builtin_fmaf(s.onePC, s.Nr, s.pq * s.Ni);
-            const float dni = __builtin_fmaf(s.onePC, s.Ni, -(s.pq * s.Nr));
-            Ur = dt * dnr;
-            Ui = dt * dni;
-            cr = __builtin_fmaf(h.ra, a.a1r, -(h.rb * b.a1r));
-            ci = __builtin_fmaf(h.ra, a.a1i, -(h.rb * b.a1i));
+            const PotRad32 b = pot_rad32(h.cx1, rem2, I2, h.rb, h.rb2, h.crb, h.icrb2);
+            const float sig = (a.d + b.d) * rcp32(a.w + b.w);
+            const float dR = h.pdr * __builtin_fmaf(-h.cx1, sig, h.pc);
+            const float arg = atan2f_lt(I * dR, __builtin_fmaf(a.R, b.R, I2));
+            const float Q2a = __builtin_fmaf(a.R, a.R, I2);
+            const float Sa = (h.cra + a.d) + v, Sb = (h.crb + b.d) + v;
+            const float ga = __builtin_fmaf(h.pomc, a.d, v);
+            const float Pa = __builtin_fmaf(ga, Sa + a.R, -I2);
+            const float dP = __builtin_fmaf(Sb + b.R, h.pomcdr * sig, -(ga * (h.pcdr * (2.f - sig))));
+            const float N = __builtin_fmaf(Pa, dR * (a.R + b.R), Q2a * dP);
+            const float ay = 0.5f * log1pf_lt(N * rcp32((Sb * Sb) * Q2a));
+            const float L = __builtin_fmaf(h.sra, a.lq, -(h.srb * b.lq));
+            return h.amp * __builtin_fmaf(x, arg, __builtin_fmaf(y, ay, -L));
         }
-        const float iV2 = rcp32(__builtin_fmaf(Vr, Vr, Vi * Vi));
-        const float z2r = __builtin_fmaf(Ur, Vr, Ui * Vi) * iV2, z2i = __builtin_fmaf(Ui, Vr, -(Ur * Vi)) * iV2;
-        const float a2r = 0.5f * log1pf_lt(__builtin_fmaf(z2i, z2i, z2r * (2.f + z2r)));
-        const float a2i = atan2f_lt(z2i, 1.f + z2r);
-        const float nr = __builtin_fmaf(s.Nr, cr, -(s.Ni * ci)), ni = __builtin_fmaf(s.Nr, ci, s.Ni * cr);
-        const float l1 = __builtin_fmaf(ni, x, -(nr * y));
-        const float l2 = __builtin_fmaf(a2i, x, -(a2r * y));
-        return h.amp * __builtin_fmaf(s.isrem, l1, l2);
     }
 }
 
 template <int TYPE, int PX>
 __device__ __forceinline__ void runp32(const HaloRec* __restrict__ h64, const PotRec32* __restrict__ p32, int start,
-                                       int end, const double* px, double py, double* s)
+                                       int end, const double* px, double py, double* s, float* mn)
 {
 #pragma clang fp contract(off)
     for (int c0 = start; c0 < end; c0 += CHUNK) {
@@ -1004,7 +986,7 @@ __device__ __forceinline__ void runp32(const HaloRec* __restrict__ h64, const Po
             const float ty = (float)(py - h64[i].yh);
 #pragma unroll
             for (int k = 0; k < PX; ++k) {
-                const float v = pot32<TYPE>(h, offset_x(xh, px[k]), ty);
+                const float v = pot32<TYPE>(h, offset_x(xh, px[k]), ty, mn[k]);
                 f[k] = f[k] + v;
             }
         }
@@ -1026,10 +1008,12 @@ __global__ __launch_bounds__(WAVE* ROWS) void k_potential32(const HaloRec* __res
     const int col0 = blockIdx.x * (WAVE * PX) + threadIdx.x;
     const double py = pixel_coord(g.ymin, row + g.jstart, g.dy);
     double px[PX], tot[PX];
+    float mn[PX];
 #pragma unroll
     for (int k = 0; k < PX; ++k) {
         px[k] = pixel_coord(g.xmin, col0 + k * WAVE + g.istart, g.dx);
         tot[k] = 0.;
+        mn[k] = 1.f;
     }
     for (int r = 0; r < nruns; ++r) {
         const Run run = runs[r];
@@ -1037,16 +1021,18 @@ __global__ __launch_bounds__(WAVE* ROWS) void k_potential32(const HaloRec* __res
 #pragma unroll
         for (int k = 0; k < PX; ++k) s[k] = 0.;
         const int end = run.start + run.count;
-        if (run.type == TYPE_DPIE) runp32<TYPE_DPIE, PX>(h64, p32, run.start, end, px, py, s);
-        else if (run.type == TYPE_PIEMD) runp32<TYPE_PIEMD, PX>(h64, p32, run.start, end, px, py, s);
-        else runp32<TYPE_SIS, PX>(h64, p32, run.start, end, px, py, s);
+        if (run.type == TYPE_DPIE) runp32<TYPE_DPIE, PX>(h64, p32, run.start, end, px, py, s, mn);
+        else if (run.type == TYPE_PIEMD) runp32<TYPE_PIEMD, PX>(h64, p32, run.start, end, px, py, s, mn);
+        else runp32<TYPE_SIS, PX>(h64, p32, run.start, end, px, py, s, mn);
 #pragma unroll
         for (int k = 0; k < PX; ++k) tot[k] += s[k];
     }
 #pragma unroll
     for (int k = 0; k < PX; ++k) {
         const int col = col0 + k * WAVE;
-        if (col < g.nx) o.p0[(int64_t)row * g.pitch + col] = tot[k] * o.mult;
+        // a pixel on a PIEMD centre: the reference's potential is 0/0 there
+        const double v = mn[k] == 0.f ? __builtin_nan("") : tot[k] * o.mult;
+        if (col < g.nx) o.p0[(int64_t)row * g.pitch + col] = v;
     }
 }
 

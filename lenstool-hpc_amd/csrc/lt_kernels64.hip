@@ -176,11 +176,12 @@ __global__ __launch_bounds__(WAVE* ROWS) void k_potential64(const HaloRec* __res
     halos = select_sample(g, halos, o);
     const int col0 = blockIdx.x * (WAVE * PX) + threadIdx.x;
     const double py = pixel_coord(g.ymin, row + g.jstart, g.dy);
-    double px[PX], tot[PX];
+    double px[PX], tot[PX], mn[PX];
 #pragma unroll
     for (int k = 0; k < PX; ++k) {
         px[k] = pixel_coord(g.xmin, col0 + k * WAVE + g.istart, g.dx);
         tot[k] = 0.;
+        mn[k] = 1.;
     }
     for (int r = 0; r < nruns; ++r) {
         const Run run = runs[r];
@@ -190,7 +191,7 @@ __global__ __launch_bounds__(WAVE* ROWS) void k_potential64(const HaloRec* __res
         for (int i = run.start; i < run.start + run.count; ++i) {
             const HaloRec& h = halos[i];
 #pragma unroll
-            for (int k = 0; k < PX; ++k) s[k] += pot_halo(h, run.type, px[k], py);
+            for (int k = 0; k < PX; ++k) s[k] += pot_halo(h, run.type, px[k], py, mn[k]);
         }
 #pragma unroll
         for (int k = 0; k < PX; ++k) tot[k] += s[k];
@@ -198,7 +199,9 @@ __global__ __launch_bounds__(WAVE* ROWS) void k_potential64(const HaloRec* __res
 #pragma unroll
     for (int k = 0; k < PX; ++k) {
         const int col = col0 + k * WAVE;
-        if (col < g.nx) o.p0[(int64_t)row * g.pitch + col] = tot[k] * o.mult;
+        // a pixel on a PIEMD centre: the reference's potential is 0/0 there
+        const double v = mn[k] == 0. ? __builtin_nan("") : tot[k] * o.mult;
+        if (col < g.nx) o.p0[(int64_t)row * g.pitch + col] = v;
     }
 }
 

@@ -70,8 +70,6 @@ int pack_model(const lt_potential_soa* L, int n, Use use, std::vector<HaloRec>& 
         h.yh = y;
         h.cs = cs;
         h.sn = sn;
-        h.b0 = b0;
-        h.eps = eps;
         const double cos2 = cs * cs - sn * sn, sin2 = 2.0 * cs * sn;
         if (t == TYPE_SIS) {
             if (!(std::fabs(eps) < 1.0)) return bad(err, i, "SIS ellipticity_potential must be in (-1, 1)");
@@ -124,11 +122,20 @@ int pack_model(const lt_potential_soa* L, int n, Use use, std::vector<HaloRec>& 
                 h.ra2c = cx1 * cx1 * rc * rc;
                 h.rb2c = cx1 * cx1 * rcut * rcut;
             }
-            // potential constants (lt_terms64.h pot_halo)
-            h.pkp = 2.0 * sqe / (1.0 - eps);
-            h.pkq = 2.0 * sqe / (1.0 + eps);
-            h.irc = 1.0 / rc;
-            h.ircut = (t == TYPE_DPIE) ? 1.0 / rcut : 0.0;
+            // potential constants (lt_terms64.h pot_piemd): c = 1 + cx1 = 2 / (1 + e)
+            const double c = 2.0 / (1.0 + eps);
+            h.psra = sqe * rc;
+            h.psrb = sqe * rcut;
+            h.pcra = c * rc;
+            h.pcrb = c * rcut;
+            h.picra = 1.0 / (c * rc);
+            h.picra2 = h.picra * h.picra;
+            h.picrb2 = (t == TYPE_DPIE) ? 1.0 / ((c * rcut) * (c * rcut)) : 0.0;
+            h.pc = c;
+            h.pomc = 1.0 - cx1;
+            h.pdr = rcut - rc;
+            h.pcdr = c * (rcut - rc);
+            h.pomcdr = (1.0 - cx1) * (rcut - rc);
         } else {  // external shear: 3 eps (cos 2t, sin 2t; sin 2t, -cos 2t)
             h.h14a = 3.0 * eps * cos2;
             h.h14b = 3.0 * eps * sin2;
@@ -233,19 +240,30 @@ int pack_model(const lt_potential_soa* L, int n, Use use, std::vector<HaloRec>& 
             if (piemd) {
                 q.kx = (float)h.kx;
                 q.ky = (float)h.ky;
-                q.pkp = (float)h.pkp;
-                q.pkq = (float)h.pkq;
+                q.cx1 = (float)h.cx1;
+                q.gki = (float)h.gki;
+                q.gkv = (float)h.gkv;
                 q.ra = (float)h.rc;
-                q.ira = (float)h.irc;
                 q.ra2 = (float)h.ra2;
+                q.cra = (float)h.pcra;
+                q.icra2 = (float)h.picra2;
+                q.icra = (float)h.picra;
+                q.sra = (float)h.psra;
                 q.rb = (float)h.rcut;
-                q.irb = (float)h.ircut;
                 q.rb2 = (float)h.rb2;
+                q.crb = (float)h.pcrb;
+                q.icrb2 = (float)h.picrb2;
+                q.srb = (float)h.psrb;
+                q.pc = (float)h.pc;
+                q.pomc = (float)h.pomc;
+                q.pdr = (float)h.pdr;
+                q.pcdr = (float)h.pcdr;
+                q.pomcdr = (float)h.pomcdr;
                 q.amp = (float)h.ampg;
             } else if (t == TYPE_SIS) {
                 q.sa = (float)h.sa;
                 q.sb = (float)h.sb;
-                q.amp = (float)h.b0;
+                q.amp = (float)h.ampg;
             }
             std::memcpy(&f, &q, sizeof(q));
         }

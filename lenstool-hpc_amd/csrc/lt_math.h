@@ -89,6 +89,14 @@ __device__ __forceinline__ double horner_step(double p, double z, double c)
     return r;
 }
 
+// min(a, b) of two non-NaN doubles as one instruction (fmin adds a quieting v_max per operand)
+__device__ __forceinline__ double min64(double a, double b)
+{
+    double r;
+    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
 __device__ __forceinline__ double atan_lt_poly(double t)
 {
     const double z = t * t;

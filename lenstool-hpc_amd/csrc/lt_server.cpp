@@ -112,7 +112,7 @@ double relative_cost(int kind, int precision)
     switch (kind) {
     case LT_MAP_GRADIENT:
     case LT_MAP_DPL: return f32 ? 0.9 : 2.3;
-    case LT_MAP_POT: return f32 ? 2.4 : 6.6;
+    case LT_MAP_POT: return f32 ? 1.4 : 4.2;
     default: return f32 ? 0.6 : 1.0;
     }
 }

@@ -231,123 +231,93 @@ __device__ __forceinline__ void grad_sis_circ(const HaloRec& h, double px, doubl
 }
 
 // ------------------------------------------------------------------------------------------------
-// potential (complex I_{0.5} potential of Kassiola & Kovner, src/potential_GPU.cu:47-72)
+// potential (the I_{0.5} potential of Kassiola & Kovner, src/potential_GPU.cu:47-72)
 // ------------------------------------------------------------------------------------------------
-// The reference's I_{0.5} potential (src/potential_GPU.cu:47-72) is written with complex
-// cosh / sinh / log of eta = -asinh(e1 y / z) / 2 + i asin(e2 x / z) / 2 and the real
-// zeta = ln((rem + sqrt(rc^2 + rem^2)) / rc) / 2. Every hyperbolic and circular function of eta and
-// zeta it uses has an algebraic value: with p = e1 y / z, q = e2 x / z, P = sqrt(1 + p^2),
-// C = sqrt(1 - q^2) and W = sqrt(rc^2 + rem^2),
-//   cosh 2 eta = P C - i p q,   sinh 2 eta = -p C + i P q,   tanh eta = sinh 2 eta / (1 + cosh 2 eta),
-//   sinh 2 zeta = rem / rc,     sinh^2 zeta = (W / rc - 1) / 2,   tanh zeta = rem / (rc + W),
-// so that (identities cosh(a+b) cosh(a-b) = cosh^2 a + sinh^2 b and cosh(a+b) / cosh(a-b) =
-// (1 + tanh a tanh b) / (1 - tanh a tanh b))
-//   ln(cosh^2 eta / (cosh(eta+zeta) cosh(eta-zeta))) = ln((1 + P C - i p q) / (P C + W / rc - i p q))
-//   ln(cosh(eta+zeta) / cosh(eta-zeta))             = ln((D + t N) / (D - t N)),
-// N = -p C + i P q, D = 1 + P C - i p q, t = tanh zeta. Two more steps make it shorter and
-// accurate next to the centre:
-//  * the first logarithm is -ln(1 + z) with z = (W / rc - 1) / D and W / rc - 1 = rem^2 / (rc (rc + W))
-//    = (rem / rc) t, formed without cancellation; ln|1 + z| = log1p(2 Re z + |z|^2) / 2. (Written as
-//    the logarithm of a quotient of two nearly equal numbers, its relative error grows as rc^2 / r^2
-//    at a distance r from the centre: about 1e-11 of the potential at r = rc / 100.)
-//  * the second logarithm a2 = ln((D + t N) / (D - t N)) = ln(1 + z2), z2 = 2 t N / (D - t N), is
-//    also taken as a log1p (its quotient tends to 1 at the centre, where t -> 0);
-//  * for the dPIE, the second logarithms of the core and cut radii enter the difference with the same
-//    weight, so they are one logarithm: ln((D + ta N)(D - tb N) / ((D - ta N)(D + tb N))) = ln(1 + z2)
-//    with z2 = 2 (ta - tb) D N / ((D - ta N)(D + tb N)). Its argument is the difference of the two
-//    arguments (no branch cut is crossed: each arg cosh(eta +- zeta) = atan(tanh(a +- zeta) tan b)
-//    lies in (-pi/4, pi/4), eta = a + i b, |b| < pi/4, so each argument difference is in
-//    (-pi/2, pi/2) and theirs in (-pi, pi)).
-// Per dPIE halo x pixel: three logarithms and three arguments instead of four of each.
-struct PotShared {
-    double p, q, PC, pq, Nr, Ni, rem2, isrem, srem, onePC, kr, ki;
+// The reference evaluates the potential of a core radius r with complex cosh / sinh / log of
+// eta = -asinh(e1 y / z) / 2 + i asin(e2 x / z) / 2 and zeta = ln((rem + sqrt(r^2 + rem^2)) / r) / 2
+// (two complex logarithms and their arguments per radius). The same function is
+//   phi_r = x alpha_x + y alpha_y - 2 sqrt(e) r ln((1 + e) |Q| / (2 r))            (halo frame)
+// with (alpha_x, alpha_y) = (arg Q, ln(S / |Q|)) the I_{0.5} deflection of the gradient (grad_piemd,
+// without its amplitude), Q = r + cx1 w + i 2 s a^2 x, S = w + r cx1 + 2 s a b y, s = sqrt(e),
+// a = 1/(1+e), b = 1/(1-e), w = sqrt(r^2 + a^2 x^2 + b^2 y^2). It is the cored isothermal-ellipsoid
+// potential (x . grad phi minus the logarithmic term that makes its Laplacian 2 kappa): both forms have
+// the same gradient, and both vanish in the limit at the centre, where Q = c r with c = 1 + cx1 =
+// 2 / (1 + e), so they are one function (they agree to a few 1e-15 relative, checked at 40 digits
+// against the extended-precision reference form by tests/test_potential_form.py). With
+// d = w - r = rem^2 / (w + r), Re Q = c r + cx1 d and S = c r + d + v (v = 2 s a b y), the logarithms
+//   lq = ln(|Q|^2 / (c r)^2) = log1p((cx1 d (2 c r + cx1 d) + (Im Q)^2) / (c r)^2),
+//   ls = ln(S / (c r))       = log1p((d + v) / (c r))
+// have arguments formed without cancellation, and phi_r = x arg Q + y (ls - lq / 2) - s r lq (PIEMD).
+// For the dPIE (core radius a minus cut radius b) the deflection parts are merged, with no difference
+// of nearly equal numbers near the centre or far outside the cut radius:
+//   * Rb - Ra = D (c - cx1 sigma), D = rcut - rcore, sigma = (da + db) / (wa + wb) in [0, 1)
+//     (db - da = -D sigma), so arg(Qa / Qb) = atan2(I (Rb - Ra), Ra Rb + I^2);
+//   * alpha_y,a - alpha_y,b = ln(Sa^2 |Qb|^2 / (Sb^2 |Qa|^2)) / 2 = log1p(N / (Sb^2 |Qa|^2)) / 2 with
+//     N = Pa (|Qb|^2 - |Qa|^2) + |Qa|^2 (Pa - Pb), P = S^2 - |Q|^2 = g (S + Re Q) - I^2,
+//     g = S - Re Q = (1 - cx1) d + v, |Qb|^2 - |Qa|^2 = (Rb - Ra)(Rb + Ra) and
+//     Pa - Pb = -ga c D (2 - sigma) + (Sb + Rb)(1 - cx1) D sigma;
+// leaving per dPIE halo x pixel one arctangent and three logarithms, against three of each in the
+// reference's form. Measured in a float64 emulation against 30-digit values, over halos of
+// e <= 0.63 (mass ellipticity up to 0.9) at distances 1e-3 to 1e3 cut radii: within 2.1e-14 of the
+// potential (5e-13 at e = 0.85, where S = w + v + cx1 r is itself a difference next to the -y axis;
+// the PIEMD form within 2.6e-14 there).
+// At the halo centre itself the reference's potential is 0/0 (NaN); this form gives its limit 0 there,
+// so the kernels track the smallest rem^2 of a pixel over its PIEMD halos and return NaN when it is 0.
+struct PotRad {
+    double w, d, R, lq;  // w, w - r, Re Q, ln(|Q|^2 / (c r)^2)
 };
 
-// pixel quantities shared by both core radii (per-halo constants from the record: the divisions
-// and square roots of eps are the packer's)
-__device__ __forceinline__ PotShared pot_shared(const HaloRec& h, const PixelFrame& f)
+__device__ __forceinline__ PotRad pot_rad(const HaloRec& h, double rem2, double I2, double r, double r2, double cr,
+                                          double icr2)
 {
-    const double iz = rsq64(f.x2 + f.y2);
-    PotShared s;
-    s.p = h.pkp * f.y * iz;
-    s.q = h.pkq * f.x * iz;
-    const double P2 = __builtin_fma(s.p, s.p, 1.), C2 = __builtin_fma(-s.q, s.q, 1.);
-    const double P = P2 * rsq64(P2), C = C2 * rsq64(C2);
-    s.PC = P * C;
-    s.pq = s.p * s.q;
-    s.Nr = -s.p * C;
-    s.Ni = P * s.q;
-    s.rem2 = __builtin_fma(f.x2, h.kx, f.y2 * h.ky);
-    s.isrem = rsq64(s.rem2);
-    s.srem = s.rem2 * s.isrem;
-    // 1 / D = conj(D) / |D|^2 = (kr + i ki) with D = 1 + PC - i pq
-    s.onePC = 1. + s.PC;
-    const double iD2 = rcp64(__builtin_fma(s.onePC, s.onePC, s.pq * s.pq));
-    s.kr = s.onePC * iD2;
-    s.ki = s.pq * iD2;
-    return s;
+    const double q = r2 + rem2;
+    PotRad o;
+    o.w = q * rsq64_n2(q);
+    o.d = rem2 * rcp64_n2(o.w + r);
+    const double cd = h.cx1 * o.d;
+    o.R = cd + cr;
+    o.lq = log1p_lt(__builtin_fma(cd, __builtin_fma(2., cr, cd), I2) * icr2);
+    return o;
 }
 
-// The core-radius part of pi05: a1 = ln(cosh^2 eta / (cosh(eta+zeta) cosh(eta-zeta))) = -ln(1 + z),
-// z = (rem / rc) t / D, and t = tanh zeta = rem / (rc + W).
-struct PotCore {
-    double a1r, a1i, t;
-};
-
-__device__ __forceinline__ PotCore pot_core(const PotShared& s, double rc, double irc)
+// pi05 of a PIEMD (type 8) or dPIE (type 81) halo without its amplitude; mn = min(mn, rem^2).
+template <bool CUT>
+__device__ __forceinline__ double pot_piemd(const HaloRec& h, const PixelFrame& f, double& mn)
 {
-    const double W2 = __builtin_fma(rc, rc, s.rem2);
-    const double W = W2 * rsq64(W2);
-    PotCore c;
-    c.t = s.srem * rcp64(rc + W);
-    const double w1 = (s.srem * irc) * c.t;
-    const double zr = w1 * s.kr, zi = w1 * s.ki;
-    c.a1r = -0.5 * log1p_lt(__builtin_fma(zi, zi, zr * (2. + zr)));
-    c.a1i = -atan2_lt(zi, 1. + zr);
-    return c;
-}
-
-// pi05 without the amplitude b0 ci, as rc / rem L(N a1) + L(a2) with L(c) = c.im x - c.re y:
-// the reference's rc / rem ((c1 + c2).im x - (c1 + c2).re y), c1 = sinh(2 eta) a1 = N a1,
-// c2 = (rem / rc) a2; for the dPIE the difference of the core and cut radii's values.
-__device__ __forceinline__ double pot_halo(const HaloRec& h, int type, double px, double py)
-{
-    PixelFrame f = halo_frame(h, px, py);
-    if (type == TYPE_SIS) return h.b0 * sqrt(__builtin_fma(h.sa, f.x2, h.sb * f.y2));
-    const PotShared s = pot_shared(h, f);
-    const PotCore a = pot_core(s, h.rc, h.irc);
-    // a2 = ln(1 + z2), z2 = U / V: PIEMD U = 2 t N, V = D - t N; dPIE U = 2 (ta - tb) D N,
-    // V = (D - ta N)(D + tb N)
-    const double amr = __builtin_fma(-a.t, s.Nr, s.onePC), ami = __builtin_fma(-a.t, s.Ni, -s.pq);
-    double cr, ci, Ur, Ui, Vr, Vi;
-    if (type == TYPE_PIEMD) {
-        Ur = 2. * a.t * s.Nr;
-        Ui = 2. * a.t * s.Ni;
-        Vr = amr;
-        Vi = ami;
-        cr = h.rc * a.a1r;
-        ci = h.rc * a.a1i;
-    } else {
-        const PotCore b = pot_core(s, h.rcut, h.ircut);
-        const double bpr = __builtin_fma(b.t, s.Nr, s.onePC), bpi = __builtin_fma(b.t, s.Ni, -s.pq);
-        Vr = __builtin_fma(amr, bpr, -(ami * bpi));
-        Vi = __builtin_fma(amr, bpi, ami * bpr);
-        const double dt = 2. * (a.t - b.t);
-        const double dnr = __builtin_fma(s.onePC, s.Nr, s.pq * s.Ni), dni = __builtin_fma(s.onePC, s.Ni, -(s.pq * s.Nr));
-        Ur = dt * dnr;
-        Ui = dt * dni;
-        cr = __builtin_fma(h.rc, a.a1r, -(h.rcut * b.a1r));
-        ci = __builtin_fma(h.rc, a.a1i, -(h.rcut * b.a1i));
+    const double rem2 = __builtin_fma(f.x2, h.kx, f.y2 * h.ky);
+    mn = min64(mn, rem2);
+    const double I = h.gki * f.x;
+    const double I2 = I * I;
+    const double v = h.gkv * f.y;
+    const PotRad a = pot_rad(h, rem2, I2, h.rc, h.ra2, h.pcra, h.picra2);
+    if (!CUT) {
+        const double ls = log1p_lt((a.d + v) * h.picra);
+        const double arg = atan2_lt(I, a.R);
+        const double ay = __builtin_fma(-0.5, a.lq, ls);
+        return __builtin_fma(f.x, arg, __builtin_fma(f.y, ay, -(h.psra * a.lq)));
     }
-    const double iV2 = rcp64(__builtin_fma(Vr, Vr, Vi * Vi));
-    const double z2r = __builtin_fma(Ur, Vr, Ui * Vi) * iV2, z2i = __builtin_fma(Ui, Vr, -(Ur * Vi)) * iV2;
-    const double a2r = 0.5 * log1p_lt(__builtin_fma(z2i, z2i, z2r * (2. + z2r)));
-    const double a2i = atan2_lt(z2i, 1. + z2r);
-    // N (cr + i ci)
-    const double nr = __builtin_fma(s.Nr, cr, -(s.Ni * ci)), ni = __builtin_fma(s.Nr, ci, s.Ni * cr);
-    const double l1 = __builtin_fma(ni, f.x, -(nr * f.y));
-    const double l2 = __builtin_fma(a2i, f.x, -(a2r * f.y));
-    return h.ampg * __builtin_fma(s.isrem, l1, l2);
+    const PotRad b = pot_rad(h, rem2, I2, h.rcut, h.rb2, h.pcrb, h.picrb2);
+    const double sig = (a.d + b.d) * rcp64_n2(a.w + b.w);
+    const double dR = h.pdr * __builtin_fma(-h.cx1, sig, h.pc);
+    const double arg = atan2_lt(I * dR, __builtin_fma(a.R, b.R, I2));
+    const double Q2a = __builtin_fma(a.R, a.R, I2);
+    const double Sa = (h.pcra + a.d) + v, Sb = (h.pcrb + b.d) + v;
+    const double ga = __builtin_fma(h.pomc, a.d, v);
+    const double Pa = __builtin_fma(ga, Sa + a.R, -I2);
+    const double dP = __builtin_fma(Sb + b.R, h.pomcdr * sig, -(ga * (h.pcdr * (2. - sig))));
+    const double N = __builtin_fma(Pa, dR * (a.R + b.R), Q2a * dP);
+    const double ay = 0.5 * log1p_lt(N * rcp64_n2((Sb * Sb) * Q2a));
+    const double L = __builtin_fma(h.psra, a.lq, -(h.psrb * b.lq));
+    return __builtin_fma(f.x, arg, __builtin_fma(f.y, ay, -L));
+}
+
+// One halo's potential at (px, py) (SIS: b0 R, R^2 = (1-e) x^2 + (1+e) y^2).
+__device__ __forceinline__ double pot_halo(const HaloRec& h, int type, double px, double py, double& mn)
+{
+    const PixelFrame f = halo_frame(h, px, py);
+    if (type == TYPE_SIS) return h.ampg * sqrt(__builtin_fma(h.sa, f.x2, h.sb * f.y2));
+    const double v = (type == TYPE_PIEMD) ? pot_piemd<false>(h, f, mn) : pot_piemd<true>(h, f, mn);
+    return h.ampg * v;
 }
 
 // ------------------------------------------------------------------------------------------------

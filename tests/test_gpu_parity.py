@@ -273,7 +273,32 @@ def test_potential_next_to_the_centre(hip, oracle, htype):
     for i in range(lens.n):
         S += np.abs(oracle.potential_grid(lens.subset([i]), grid, extended=True))
     assert np.isfinite(phi).all()
-    assert_close("phi", phi, ext, S, np.zeros_like(S))
+    r = assert_close("phi", phi, ext, S, np.zeros_like(S))
+    print(f"[potential next to a type-{htype} centre] max error / tolerance {r:.3e}", flush=True)
+
+
+@pytest.mark.parametrize("epot", [0.3, 0.6])
+def test_potential_lone_dpie_far_and_near(hip, oracle, epot):
+    """One dPIE alone (so the criterion is 1e-12 of its own |phi|): pixels 30 to 300 cut radii out,
+    where the core and cut radii's deflections nearly cancel, and pixels 0.02 to 0.5 core radii from
+    the centre, where every term vanishes; both merged parts are formed without that cancellation."""
+    lens = M.LensModel(type=np.array([81], np.int32), x=np.array([0.4]), y=np.array([-0.3]),
+                       b0=np.array([12.0]), angle=np.array([0.7]), epot=np.array([epot]),
+                       rcore=np.array([0.2]), rcut=np.array([1.5]), z=np.array([0.4]), dlsds=np.array([1.0]))
+    for lo, hi in ((45.0, 450.0), (-450.0, -45.0)):
+        grid = M.Grid(M.grid_param(lo, hi, lo, hi), 48, 40)
+        print_case(f"lone dPIE e={epot} far {lo}..{hi}", lens, grid)
+        phi = hip.potential_grid(lens, grid)
+        ext = oracle.potential_grid(lens, grid, extended=True)
+        r = assert_close("phi far", phi, ext, np.abs(ext), np.zeros_like(ext))
+        print(f"[lone dPIE e={epot} far] max error / tolerance {r:.3e}", flush=True)
+    x0, y0, step, n = 0.4 + 0.004, -0.3 + 0.004, 0.1 / 40, 40
+    grid = M.Grid(M.grid_param(x0, x0 + (n - 1) * step, y0, y0 + (n - 1) * step), n, n)
+    print_case(f"lone dPIE e={epot} near the centre", lens, grid)
+    phi = hip.potential_grid(lens, grid)
+    ext = oracle.potential_grid(lens, grid, extended=True)
+    r = assert_close("phi near", phi, ext, np.abs(ext), np.zeros_like(ext))
+    print(f"[lone dPIE e={epot} near] max error / tolerance {r:.3e}", flush=True)
 
 
 @pytest.mark.parametrize("which", ["mixed", "cluster"])
@@ -294,7 +319,8 @@ def test_potential(hip, oracle, which):
     for i in range(lens.n):
         S += np.abs(oracle.potential_grid(lens.subset([i]), grid, extended=True))
     assert np.isfinite(phi).all()
-    assert_close("phi", phi, ext, S, np.zeros_like(S))
+    r = assert_close("phi", phi, ext, S, np.zeros_like(S))
+    print(f"[potential {which}] max error / tolerance {r:.3e}", flush=True)
     # 1, 2 and 4 pixels per thread give the same map bit for bit
     m = hip.Model(lens, hip.MAP_POT)
     out = hip.DeviceArray(grid.npix)
