@@ -949,14 +949,14 @@ __device__ __forceinline__ float pot32(const PotRec32& h, float tx, float ty, fl
         const PotRad32 a = pot_rad32(h.cx1, rem2, I2, h.ra, h.ra2, h.cra, h.icra2);
         if constexpr (TYPE == TYPE_PIEMD) {
             const float ls = log1pf_lt((a.d + v) * h.icra);
-            const float arg = atan2f_lt(I, a.R);
+            const float arg = atan2pf_lt(I, a.R);
             const float ay = __builtin_fmaf(-0.5f, a.lq, ls);
             return h.amp * __builtin_fmaf(x, arg, __builtin_fmaf(y, ay, -(h.sra * a.lq)));
         } else {
             const PotRad32 b = pot_rad32(h.cx1, rem2, I2, h.rb, h.rb2, h.crb, h.icrb2);
             const float sig = (a.d + b.d) * rcp32(a.w + b.w);
             const float dR = h.pdr * __builtin_fmaf(-h.cx1, sig, h.pc);
-            const float arg = atan2f_lt(I * dR, __builtin_fmaf(a.R, b.R, I2));
+            const float arg = atan2pf_lt(I * dR, __builtin_fmaf(a.R, b.R, I2));
             const float Q2a = __builtin_fmaf(a.R, a.R, I2);
             const float Sa = (h.cra + a.d) + v, Sb = (h.crb + b.d) + v;
             const float ga = __builtin_fmaf(h.pomc, a.d, v);

@@ -124,6 +124,24 @@ __device__ __forceinline__ double atan2_lt(double y, double x)
     return __builtin_isunordered(x, y) ? x + y : r;
 }
 
+// atan2(y, x) for x > 0, the case of the I_{0.5} terms (their x is Re Q > 0 or Ra Rb + I^2 > 0): the
+// angle lies in (-pi/2, pi/2), so atan2_lt's quadrant, zero and NaN cases drop out; the same folding
+// and polynomial otherwise.
+__device__ __forceinline__ double atan2p_lt(double y, double x)
+{
+    constexpr double kPi2 = 1.5707963267948965580, kPi4 = 0.78539816339744827900;
+    constexpr double kTanPi8 = 0.41421356237309503;
+    const double ay = __builtin_fabs(y);
+    const double mx = __builtin_fmax(x, ay), mn = __builtin_fmin(x, ay);
+    const bool rot = mn > kTanPi8 * mx;
+    const double u = rot ? mx + mn : mx;
+    const double v = rot ? mn - mx : mn;
+    double r = atan_lt_poly(v * rcp64(u));
+    r = rot ? r + kPi4 : r;
+    r = ay > x ? kPi2 - r : r;
+    return __builtin_copysign(r, y);
+}
+
 // FP32 arctangent for the FP32-compensated gradient: the folding of atan2_lt, then
 // atan(t) = t + t^3 P(t^2) with P of degree 4 (kAtanP32, gen_poly.py; interpolation error 1.1e-9
 // in atan, below FP32 rounding); finite arguments.
@@ -153,6 +171,26 @@ __device__ __forceinline__ float atan2f_lt(float y, float x)
     r = ay > ax ? kPi2 - r : r;
     r = __builtin_signbitf(x) ? kPi - r : r;
     r = mx == 0.f ? (__builtin_signbitf(x) ? kPi : 0.f) : r;
+    return __builtin_copysignf(r, y);
+}
+
+// atan2f_lt for x > 0 (as atan2p_lt)
+__device__ __forceinline__ float atan2pf_lt(float y, float x)
+{
+    constexpr float kPi2 = 1.57079633f, kPi4 = 0.785398163f, kTanPi8 = 0.414213562f;
+    const float ay = __builtin_fabsf(y);
+    const float mx = __builtin_fmaxf(x, ay), mn = __builtin_fminf(x, ay);
+    const bool rot = mn > kTanPi8 * mx;
+    const float u = rot ? mx + mn : mx;
+    const float v = rot ? mn - mx : mn;
+    const float t = v * __builtin_amdgcn_rcpf(u);
+    const float z = t * t;
+    float p = kAtanP32[4];
+#pragma unroll
+    for (int k = 3; k >= 0; --k) p = __builtin_fmaf(p, z, kAtanP32[k]);
+    float r = __builtin_fmaf(t * z, p, t);
+    r = rot ? r + kPi4 : r;
+    r = ay > x ? kPi2 - r : r;
     return __builtin_copysignf(r, y);
 }
 

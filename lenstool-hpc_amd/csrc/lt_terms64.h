@@ -173,7 +173,7 @@ __device__ __forceinline__ QS64 qs64(const HaloRec& h, double rem2, double I2, d
                                      double rcx)
 {
     const double q = r2 + rem2;
-    const double w = q * rsq64(q);
+    const double w = q * rsq64_n2(q);
     QS64 o;
     o.R = __builtin_fma(h.cx1, w, r);
     o.Q2 = __builtin_fma(o.R, o.R, I2);
@@ -197,11 +197,11 @@ __device__ __forceinline__ void grad_piemd(const HaloRec& h, double px, double p
     double arg, ratio;
     if (CUT) {
         const QS64 b = qs64(h, rem2, I2, vy, h.rcut, h.rb2, h.rcb);
-        arg = atan2_lt(I * (b.R - a.R), __builtin_fma(a.R, b.R, I2));
-        ratio = (a.Q2 * (b.S * b.S)) * rcp64(b.Q2 * (a.S * a.S));
+        arg = atan2p_lt(I * (b.R - a.R), __builtin_fma(a.R, b.R, I2));
+        ratio = (a.Q2 * (b.S * b.S)) * rcp64_n2(b.Q2 * (a.S * a.S));
     } else {
-        arg = atan2_lt(I, a.R);
-        ratio = a.Q2 * rcp64(a.S * a.S);
+        arg = atan2p_lt(I, a.R);
+        ratio = a.Q2 * rcp64_n2(a.S * a.S);
     }
     const double lnr = 0.5 * log_lt(ratio);
     const double re = h.ampg * arg;
@@ -292,14 +292,14 @@ __device__ __forceinline__ double pot_piemd(const HaloRec& h, const PixelFrame& 
     const PotRad a = pot_rad(h, rem2, I2, h.rc, h.ra2, h.pcra, h.picra2);
     if (!CUT) {
         const double ls = log1p_lt((a.d + v) * h.picra);
-        const double arg = atan2_lt(I, a.R);
+        const double arg = atan2p_lt(I, a.R);
         const double ay = __builtin_fma(-0.5, a.lq, ls);
         return __builtin_fma(f.x, arg, __builtin_fma(f.y, ay, -(h.psra * a.lq)));
     }
     const PotRad b = pot_rad(h, rem2, I2, h.rcut, h.rb2, h.pcrb, h.picrb2);
     const double sig = (a.d + b.d) * rcp64_n2(a.w + b.w);
     const double dR = h.pdr * __builtin_fma(-h.cx1, sig, h.pc);
-    const double arg = atan2_lt(I * dR, __builtin_fma(a.R, b.R, I2));
+    const double arg = atan2p_lt(I * dR, __builtin_fma(a.R, b.R, I2));
     const double Q2a = __builtin_fma(a.R, a.R, I2);
     const double Sa = (h.pcra + a.d) + v, Sb = (h.pcrb + b.d) + v;
     const double ga = __builtin_fma(h.pomc, a.d, v);
