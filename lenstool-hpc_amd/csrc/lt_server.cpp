@@ -111,8 +111,8 @@ double relative_cost(int kind, int precision)
     const bool f32 = precision == LT_PREC_FP32C;
     switch (kind) {
     case LT_MAP_GRADIENT:
-    case LT_MAP_DPL: return f32 ? 0.9 : 2.3;
-    case LT_MAP_POT: return f32 ? 1.4 : 4.2;
+    case LT_MAP_DPL: return f32 ? 0.85 : 2.0;
+    case LT_MAP_POT: return f32 ? 1.35 : 3.9;
     default: return f32 ? 0.6 : 1.0;
     }
 }
