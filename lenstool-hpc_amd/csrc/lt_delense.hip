@@ -100,11 +100,14 @@ __device__ __forceinline__ void record_hit(const DelenseDesc& d, int s, int k, i
 }
 
 // ------------------------------------------------------------------------------------------------
-// delensing scan: one thread per (cell, sample), all sources (staged in LDS, SCAN_BLOCK per pass);
-// hits appended per (sample, source)
+// delensing scan: one thread per (cell, sample), a wavefront per 8 x 8 tile of cells, all sources
+// (staged in LDS, SCAN_BLOCK per pass); hits appended per (sample, source)
 // ------------------------------------------------------------------------------------------------
-constexpr int SCAN_BLOCK = 256;  // cells per block along a row; also the sources staged in LDS per pass
-constexpr int SCAN_UNROLL = 4;   // sources culled per step of the scan loop
+constexpr int SCAN_BLOCK = 256;  // threads per block (4 wavefronts: 16 x 16 cells); also the sources staged in LDS per pass
+constexpr int SCAN_SIDE = 16;    // cells per block side
+constexpr int SCAN_UNROLL = 4;   // sources culled per step of the per-cell scan loop
+constexpr int WAVE_SIZE = 64;
+static_assert(SCAN_BLOCK == 4 * WAVE_SIZE && SCAN_SIDE == 16, "four wavefronts of 8 x 8 cells per block");
 
 // One pass over cnt staged sources. SCAN_UNROLL sources per step: one margin (the smallest of their
 // culls) decides the step in straight-line code; the reference test runs only where a cull fails (a
@@ -143,24 +146,64 @@ __device__ __forceinline__ void scan_pass(const DelenseDesc& d, const CellCorner
             record_hit(d, s, k0 + j0, cell, cell_hit(c, lsx[j0], lsy[j0], ldr[j0]));
 }
 
+
+
+// One pass over cnt staged sources with the tile pre-test (lt_delense_geom.h tile_filter32): lane j
+// tests source kc + j against the whole tile, and only the sources it keeps (few: those whose image lies
+// on or next to the tile) go through the per-cell tests, one source at a time for all cells.
+__device__ __forceinline__ void scan_tile(const DelenseDesc& d, const CellCorners& c, const CellFilter& f,
+                                          const CellFilter32& f32, const CellFilter32& tf, bool active, int s, int k0,
+                                          int cell, int cnt, const double* lsx, const double* lsy, const double* ldr,
+                                          const float* lsx32, const float* lsy32, const float* ldr32)
+{
+    const int lane = (int)threadIdx.x & (WAVE_SIZE - 1);
+    for (int kc = 0; kc < cnt; kc += WAVE_SIZE) {
+        const int j = kc + lane;  // < SCAN_BLOCK: kc < cnt <= SCAN_BLOCK is a multiple of the wavefront size
+        const float mg = j < cnt ? far_margin_rho32(tf, lsx32[j], lsy32[j], ldr32[j]) : 1.f;
+        unsigned long long keep = __ballot(!(mg > 0.f));
+        while (keep) {
+            const int jj = kc + __builtin_ctzll(keep);
+            keep &= keep - 1;
+            if (active && !(far_margin_rho32(f32, lsx32[jj], lsy32[jj], ldr32[jj]) > 0.f) &&
+                !far_from_cell(f, c, lsx[jj], lsy[jj], ldr[jj]))
+                record_hit(d, s, k0 + jj, cell, cell_hit(c, lsx[jj], lsy[jj], ldr[jj]));
+        }
+    }
+}
+
 __global__ __launch_bounds__(SCAN_BLOCK) void k_delense(const double* __restrict__ gx, const double* __restrict__ gy,
                                                         DelenseDesc d)
 {
     __shared__ double lsx[SCAN_BLOCK], lsy[SCAN_BLOCK], ldr[SCAN_BLOCK];
     __shared__ float lsx32[SCAN_BLOCK], lsy32[SCAN_BLOCK], ldr32[SCAN_BLOCK];
     const int n = d.n;
-    const int xid = blockIdx.x * SCAN_BLOCK + threadIdx.x;
-    const int yid = blockIdx.y;  // the grid has n - 1 rows of blocks
+    const int lane = (int)threadIdx.x & (WAVE_SIZE - 1), wv = (int)threadIdx.x / WAVE_SIZE;
+    const int xid = blockIdx.x * SCAN_SIDE + (wv & 1) * 8 + (lane & 7);
+    const int yid = blockIdx.y * SCAN_SIDE + (wv >> 1) * 8 + (lane >> 3);
     const int s = blockIdx.z;
-    const bool active = xid < n - 1;
+    const bool active = xid < n - 1 && yid < n - 1;
     const int64_t base = (int64_t)s * d.grid_stride;
     const int cell = yid * (n - 1) + xid;
-    // lanes past the row end read the last cell (in bounds) and record nothing
-    const CellCorners c = load_corners(gx + base, gy + base, d, active ? xid : n - 2, yid);
+    // lanes past the grid read the last cell of their row or column (in bounds) and record nothing
+    const CellCorners c = load_corners(gx + base, gy + base, d, min(xid, n - 2), min(yid, n - 2));
     const double* sb = d.src_bound + 3 * (int64_t)s;
     const CellFilter f = cell_filter(c, sb[0], sb[1], sb[2]);
     const CellFilter32 f32 = cell_filter32(f, c);
+    const unsigned long long act = __ballot(active);
     const bool rho_only = __all(!active || f.d_ok);  // uniform over the wavefront
+    // the tile pre-test when every active cell has d_ok: bounds about the cell at the tile's centre
+    // (lane 36, cell (4, 4) of the tile) when it is in the grid, else the first active lane's
+    CellFilter32 tf = f32;
+    tf.rf2 = __builtin_inff();
+    if (rho_only && act) {
+        const int r = ((act >> 36) & 1ull) ? 36 : __builtin_ctzll(act);
+        const double xr = read_lane64(c.xp, r), yr = read_lane64(c.yp, r), gxr = read_lane64(c.g0x, r),
+                     gyr = read_lane64(c.g0y, r);
+        const double drmax = __builtin_fmax(__builtin_fabs(sb[1]), __builtin_fabs(sb[2]));
+        const double t = active ? tile_reach(f, c, xr, yr, gxr, gyr, drmax) : 0.;
+        tf = tile_filter32(xr, yr, gxr, gyr, wave_max64(t), read_lane64(f.m, r), true);
+    }
+    const bool tile = tf.rf2 < __builtin_inff();  // uniform over the wavefront
     for (int k0 = 0; k0 < d.nsets; k0 += SCAN_BLOCK) {
         const int cnt = min(SCAN_BLOCK, d.nsets - k0);
         __syncthreads();  // the previous pass is done with the staged sources
@@ -175,7 +218,9 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_delense(const double* __restrict
             ldr32[threadIdx.x] = (float)sdr;
         }
         __syncthreads();
-        if (rho_only) scan_pass<true>(d, c, f, f32, active, s, k0, cell, cnt, lsx, lsy, ldr, lsx32, lsy32, ldr32);
+        if (!act) continue;  // a wavefront past the grid: nothing to scan (it still stages and syncs)
+        if (tile) scan_tile(d, c, f, f32, tf, active, s, k0, cell, cnt, lsx, lsy, ldr, lsx32, lsy32, ldr32);
+        else if (rho_only) scan_pass<true>(d, c, f, f32, active, s, k0, cell, cnt, lsx, lsy, ldr, lsx32, lsy32, ldr32);
         else scan_pass<false>(d, c, f, f32, active, s, k0, cell, cnt, lsx, lsy, ldr, lsx32, lsy32, ldr32);
     }
 }
@@ -343,7 +388,8 @@ hipError_t launch_delense(const double* gx, const double* gy, const DelenseDesc&
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     dim3 blk(SCAN_BLOCK);
-    dim3 grd((unsigned)((d.n - 1 + SCAN_BLOCK - 1) / SCAN_BLOCK), (unsigned)(d.n - 1), (unsigned)d.nsamples);
+    dim3 grd((unsigned)((d.n - 1 + SCAN_SIDE - 1) / SCAN_SIDE), (unsigned)((d.n - 1 + SCAN_SIDE - 1) / SCAN_SIDE),
+             (unsigned)d.nsamples);
     hipLaunchKernelGGL(k_delense, grd, blk, 0, s, gx, gy, d);
     e = hipGetLastError();
     if (e != hipSuccess) return e;

@@ -97,7 +97,8 @@ __host__ __device__ inline int cell_hit(const CellCorners& c, double px, double 
 // an end.
 struct CellFilter {
     double c0, c1u, c2u, c1l, c2l, rn2, tau;
-    double m;  // the coordinate bound M
+    double rn;  // an upper bound of sqrt(rn2)
+    double m;   // the coordinate bound M
     bool on;
     bool d_ok;  // |d| > tau for every dr of the range (both triangles): the margin needs only rho
 };
@@ -105,6 +106,16 @@ struct CellFilter {
 __host__ __device__ inline double abs_d(double v) { return __builtin_fabs(v); }
 // fmax drops NaN; the filter checks finiteness separately before it relies on these maxima
 __host__ __device__ inline double max_d(double a, double b) { return __builtin_fmax(a, b); }
+
+// An upper bound of sqrt(q) for q >= 0 from the FP32 square root (relative error below 2^-22, the
+// rounding of q to FP32 within 2^-24), enlarged by 1e-6 relative; q below 1e-30 counts as 1e-30 (the
+// bound stays a normal FP32 number; a bound of 1e-15 is far below any length these bounds compare
+// with). Four instructions instead of the FP64 square root's dozen; +infinity when q exceeds the FP32
+// range, which switches the tests that use it off.
+__host__ __device__ inline double sqrt_up(double q)
+{
+    return (double)__builtin_sqrtf((float)__builtin_fmax(q, 1e-30)) * (1. + 1e-6);
+}
 
 __host__ __device__ inline double corner_dist2(double px, double py, double gx, double gy, double dr)
 {
@@ -146,8 +157,9 @@ __host__ __device__ inline CellFilter cell_filter(const CellCorners& c, double p
         r2 = max_d(r2, corner_dist2(hx, 0., G2x, G2y, dr));
         r2 = max_d(r2, corner_dist2(hx, hy, G3x, G3y, dr));
     }
-    const double r = __builtin_sqrt(r2) * 1.000001 + 1e-9 * m;
+    const double r = sqrt_up(r2) * 1.000001 + 1e-9 * m;
     f.rn2 = (1.0011 * r) * (1.0011 * r);
+    f.rn = (1.0011 * r) * (1. + 1e-15);
     // |d| over the whole range: with dr = mid + t, |t| <= w, the quadratic is exactly
     // d(mid) + d'(mid) t + c2 t^2, so |d| >= |d(mid)| - |d'(mid)| w - |c2| w^2 (the factors 1 + 1e-12
     // and the slack 1e-13 M^2 cover the rounding of these few operations)
@@ -207,7 +219,7 @@ __host__ __device__ inline CellFilter32 cell_filter32(const CellFilter& f, const
     g.g0x = (float)c.g0x;
     g.g0y = (float)c.g0y;
     const bool ok = f.d_ok && f.m <= 1e30;
-    const double rf = __builtin_sqrt(f.rn2) + 1e-6 * f.m;
+    const double rf = f.rn + 1e-6 * f.m;
     g.rf2 = ok ? (float)(rf * rf * (1. + 1e-6)) : __builtin_inff();
     return g;
 }
@@ -218,6 +230,42 @@ __host__ __device__ inline float far_margin_rho32(const CellFilter32& g, float p
     const float ex = px - __builtin_fmaf(-dr, g.g0x, g.xp);
     const float ey = py - __builtin_fmaf(-dr, g.g0y, g.yp);
     return __builtin_fmaf(ex, ex, ey * ey) - g.rf2;
+}
+
+// Tile-level pre-test: one FP32 test per (tile of cells, source) instead of one per (cell, source).
+// For the cells i of a tile, all with d_ok, a reference cell r of the tile, and the exact differences
+// delta_i = p_i - p_r, gamma_i = g0_i - g0_r of their corner g (position and gradient), the exact
+// distance from a source P to cell i's mapped corner v_i(dr) = p_i - dr g0_i is at least
+// |P - v_r(dr)| - |delta_i| - |dr| |gamma_i|. So if
+//   |P - v_r(dr)| > T >= T_i = sqrt(rn2_i) + |delta_i| + drmax |gamma_i|   for every cell i,
+// with drmax >= |dr| for every source of the sample, every cell's distance margin holds in exact
+// arithmetic (rho_i^2 > rn2_i) and, with d_ok, both triangle tests of every cell of the tile fail.
+// tile_reach() forms T_i from upper bounds of the two norms (sqrt_up) and of sqrt(rn2) (rn), with a few
+// roundings (within 8 u of the sum of those bounds); tile_filter32() enlarges
+// the tile's maximum T by 1e-12 relative and builds the FP32 test of far_margin_rho32() about the
+// reference cell's corner, with that test's own slack (1e-6 M_r, M_r the reference cell's M, which
+// bounds every FP32 input: its corner, dr |g0| and the sources) and relative enlargement: a positive
+// margin implies |P - v_r(dr)| > T exactly. Off (rf2 = +infinity) unless ok and M_r <= 1e30.
+__host__ __device__ inline double tile_reach(const CellFilter& f, const CellCorners& c, double xr, double yr,
+                                             double gxr, double gyr, double drmax)
+{
+    const double ax = c.xp - xr, ay = c.yp - yr, bx = c.g0x - gxr, by = c.g0y - gyr;
+    const double dx = sqrt_up(__builtin_fma(ax, ax, ay * ay));
+    const double dg = sqrt_up(__builtin_fma(bx, bx, by * by));
+    return __builtin_fma(drmax, dg, f.rn + dx);
+}
+
+__host__ __device__ inline CellFilter32 tile_filter32(double xr, double yr, double gxr, double gyr, double T, double Mr,
+                                                      bool ok)
+{
+    CellFilter32 g;
+    g.xp = (float)xr;
+    g.yp = (float)yr;
+    g.g0x = (float)gxr;
+    g.g0y = (float)gyr;
+    const double rf = T * (1. + 1e-12) + 1e-6 * Mr;
+    g.rf2 = (ok && Mr <= 1e30) ? (float)(rf * rf * (1. + 1e-6)) : __builtin_inff();
+    return g;
 }
 
 // true: both of cell_hit()'s triangle tests provably fail for source (px, py, dr), by the near or the

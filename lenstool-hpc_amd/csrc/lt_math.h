@@ -97,6 +97,35 @@ __device__ __forceinline__ double min64(double a, double b)
     return r;
 }
 
+// Largest value of v over the 64 lanes of a wavefront, returned to every lane; every lane of the
+// wavefront must be executing. Within each row of 16 lanes by data-parallel moves (partner lane ^ 1,
+// ^ 2, then the mirrored lane of the half row and of the row: each step pairs lanes holding the maxima
+// of disjoint groups), then the four rows' maxima read from lanes 0, 16, 32 and 48.
+template <int CTRL>
+__device__ __forceinline__ double dpp_move64(double v)
+{
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double read_lane64(double v, int r)
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), r);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), r);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double wave_max64(double v)
+{
+    v = __builtin_fmax(v, dpp_move64<0xB1>(v));   // quad_perm [1, 0, 3, 2]
+    v = __builtin_fmax(v, dpp_move64<0x4E>(v));   // quad_perm [2, 3, 0, 1]
+    v = __builtin_fmax(v, dpp_move64<0x141>(v));  // row_half_mirror
+    v = __builtin_fmax(v, dpp_move64<0x140>(v));  // row_mirror
+    return __builtin_fmax(__builtin_fmax(read_lane64(v, 0), read_lane64(v, 16)),
+                          __builtin_fmax(read_lane64(v, 32), read_lane64(v, 48)));
+}
+
 __device__ __forceinline__ double atan_lt_poly(double t)
 {
     const double z = t * t;

@@ -11,7 +11,10 @@
 //   4. the FP32 pre-test (d_ok && far_margin_rho32() > 0) is sound on its own: whenever it skips,
 //      cell_hit() finds no hit and the exact distance from the source to the cell's corner g exceeds
 //      sqrt(rn2), which is what the FP64 margin establishes; on lens-like far pairs it decides about as
-//      often as the FP64 margin.
+//      often as the FP64 margin;
+//   5. the tile pre-test (tile_reach / tile_filter32 over an 8 x 8 tile of cells, all with d_ok): whenever
+//      it skips a source, no cell of the tile has a hit and every cell's exact corner distance exceeds
+//      sqrt(rn2); on lens-like tiles it skips most sources that no cell of the tile is near.
 // Cells come from smooth lens-like gradient fields, near-collinear and collapsed mapped triangles,
 // wild and huge gradients, sources on corners and edges, and NaN / infinity in gradients, sources and
 // D_ls/D_s. Exit status 0 = all checks hold; the counts are printed.
@@ -117,6 +120,129 @@ static void check(Stats& st, const CellCorners& c, double px, double py, double 
         const q du_x = qdet(v0, v1, v2), dl_x = qdet(v3, v2, v1);
         const q need = (q)3.2e-11 * mp2;
         if ((du_x < 0 ? -du_x : du_x) <= need || (dl_x < 0 ? -dl_x : dl_x) <= need) ++st.dok_bad;
+    }
+}
+
+struct TileStats {
+    long tiles = 0, tiles_on = 0, pairs = 0, skipped = 0, unsound = 0, dist_bad = 0, far_pairs = 0, far_skipped = 0;
+};
+
+// Check 5: an 8 x 8 tile of cells on a (9 x 9)-corner grid of spacing h in a lens-like deflection field
+// (regime 0: smooth; 1: with a strong local perturbation; 2: some corners collapsed onto a point),
+// 12 sources of one sample; the tile test against every cell's reference test and exact distance.
+static void check_tile(TileStats& ts, std::mt19937_64& rng)
+{
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    std::normal_distribution<double> N(0.0, 1.0);
+    const int regime = (int)(U(rng) * 3);
+    const double h = std::pow(10.0, -1.5 + 1.8 * U(rng));
+    const double x0 = (U(rng) - 0.5) * 200.0, y0 = (U(rng) - 0.5) * 200.0;
+    const double a = 0.9 * (2 * U(rng) - 1), b = 0.5 * (2 * U(rng) - 1), cc = 0.5 * (2 * U(rng) - 1),
+                 e = 0.9 * (2 * U(rng) - 1);
+    const double bx = 20 * N(rng), by = 20 * N(rng), kx = 2e-3 * N(rng), ky = 2e-3 * N(rng);
+    double GX[9][9], GY[9][9], X[9], Y[9];
+    for (int i = 0; i < 9; ++i) {
+        X[i] = x0 + i * h;
+        Y[i] = y0 + i * h;
+    }
+    const double vx = (U(rng) - 0.5) * 50, vy = (U(rng) - 0.5) * 50;
+    for (int j = 0; j < 9; ++j)
+        for (int i = 0; i < 9; ++i) {
+            const double u = X[i] - x0, v = Y[j] - y0;
+            GX[j][i] = bx + a * u + b * v + kx * u * u + 0.2 * x0;
+            GY[j][i] = by + cc * u + e * v + ky * v * v + 0.2 * y0;
+            if (regime == 1) {
+                const double r2 = (u - 4 * h) * (u - 4 * h) + (v - 4 * h) * (v - 4 * h) + 0.3 * h * h;
+                GX[j][i] += 3 * h * h * (u - 4 * h) / r2;
+                GY[j][i] += 3 * h * h * (v - 4 * h) / r2;
+            }
+            if (regime == 2 && U(rng) < 0.1) {  // this corner maps onto (vx, vy) at dr = 1
+                GX[j][i] = X[i] - vx;
+                GY[j][i] = Y[j] - vy;
+            }
+        }
+    // sources of the sample: near the mapped tile, or anywhere
+    const int ns = 12;
+    double sx[ns], sy[ns], sd[ns];
+    double pmax = 0, lo = 0, hi = 0;
+    for (int k = 0; k < ns; ++k) {
+        sd[k] = 0.2 + 1.2 * U(rng);
+        const int ci = (int)(U(rng) * 9), cj = (int)(U(rng) * 9);
+        const double mx = X[ci] - sd[k] * GX[cj][ci], my = Y[cj] - sd[k] * GY[cj][ci];
+        const double r = U(rng);
+        const double span = 9 * h * (1 + std::fabs(a) + std::fabs(e));
+        if (r < 0.3) {
+            sx[k] = mx + span * (U(rng) - 0.5) * 0.5;
+            sy[k] = my + span * (U(rng) - 0.5) * 0.5;
+        } else if (r < 0.7) {
+            sx[k] = mx + span * (U(rng) - 0.5) * 6;
+            sy[k] = my + span * (U(rng) - 0.5) * 6;
+        } else {
+            sx[k] = (U(rng) - 0.5) * 200;
+            sy[k] = (U(rng) - 0.5) * 200;
+        }
+        pmax = std::fmax(pmax, std::fmax(std::fabs(sx[k]), std::fabs(sy[k])));
+        lo = k ? std::fmin(lo, sd[k]) : sd[k];
+        hi = k ? std::fmax(hi, sd[k]) : sd[k];
+    }
+    CellCorners cs[64];
+    CellFilter fs[64];
+    bool all_ok = true;
+    for (int l = 0; l < 64; ++l) {
+        const int i = l & 7, j = l >> 3;
+        CellCorners& c = cs[l];
+        c.xp = X[i];
+        c.yp = Y[j];
+        c.xq = X[i + 1];
+        c.yq = Y[j + 1];
+        c.g0x = GX[j][i];
+        c.g0y = GY[j][i];
+        c.g1x = GX[j + 1][i];
+        c.g1y = GY[j + 1][i];
+        c.g2x = GX[j][i + 1];
+        c.g2y = GY[j][i + 1];
+        c.g3x = GX[j + 1][i + 1];
+        c.g3y = GY[j + 1][i + 1];
+        fs[l] = lt::cell_filter(c, pmax, lo, hi);
+        all_ok = all_ok && fs[l].d_ok;
+    }
+    ++ts.tiles;
+    // reference cell: a random one (the kernel takes the first active lane)
+    const int r = (int)(U(rng) * 64);
+    const double drmax = std::fmax(std::fabs(lo), std::fabs(hi));
+    double T = 0;
+    for (int l = 0; l < 64; ++l)
+        T = std::fmax(T, lt::tile_reach(fs[l], cs[l], cs[r].xp, cs[r].yp, cs[r].g0x, cs[r].g0y, drmax));
+    const lt::CellFilter32 tf = lt::tile_filter32(cs[r].xp, cs[r].yp, cs[r].g0x, cs[r].g0y, T, fs[r].m, all_ok);
+    ts.tiles_on += all_ok;
+    for (int k = 0; k < ns; ++k) {
+        ++ts.pairs;
+        const bool skip = lt::far_margin_rho32(tf, (float)sx[k], (float)sy[k], (float)sd[k]) > 0.f;
+        bool any_hit = false, any_near = false;
+        for (int l = 0; l < 64; ++l) {
+            const CellCorners& c = cs[l];
+            if (lt::cell_hit(c, sx[k], sy[k], sd[k])) any_hit = true;
+            if (!(fs[l].d_ok && lt::far_margin_rho(fs[l], c, sx[k], sy[k], sd[k]) > 0.)) any_near = true;
+            if (skip) {
+                const q ex = (q)sx[k] - ((q)c.xp - (q)sd[k] * c.g0x), ey = (q)sy[k] - ((q)c.yp - (q)sd[k] * c.g0y);
+                if (!(ex * ex + ey * ey > (q)fs[l].rn2)) ++ts.dist_bad;
+            }
+        }
+        ts.skipped += skip;
+        if (skip && any_hit) {
+            if (ts.unsound < 5) std::printf("UNSOUND TILE: regime %d h %.3g source (%.17g %.17g) dr %.17g\n", regime, h, sx[k], sy[k], sd[k]);
+            ++ts.unsound;
+        }
+        // a source that every cell's own FP64 margin rejects by a wide distance: the tile test should too
+        if (all_ok && regime == 0 && !any_near) {
+            double mind = 1e300;
+            for (int l = 0; l < 64; ++l)
+                mind = std::fmin(mind, std::hypot(sx[k] - (cs[l].xp - sd[k] * cs[l].g0x), sy[k] - (cs[l].yp - sd[k] * cs[l].g0y)));
+            if (mind > 2 * T) {
+                ++ts.far_pairs;
+                ts.far_skipped += skip;
+            }
+        }
     }
 }
 
@@ -242,6 +368,8 @@ int main(int argc, char** argv)
         }
         check(st, c, px, py, dr, pmax, lo, hi, lens_far);
     }
+    TileStats ts;
+    for (long t = 0; t < n / 64; ++t) check_tile(ts, rng);
     std::printf("trials %ld culled %ld hits %ld unsound %ld\n", st.trials, st.culled, st.hits, st.unsound);
     std::printf("tri_det error worst %.3g M^2 (bound 4.0e-15, over: %ld); d(dr) error worst %.3g M^2 (bound 3.2e-14, over: %ld)\n",
                 st.det_worst, st.det_bad, st.poly_worst, st.poly_bad);
@@ -251,7 +379,12 @@ int main(int argc, char** argv)
                 st.far_pairs ? (double)st.far_culled / st.far_pairs : 0.0, st.far_rho, st.far_culled32,
                 st.far_rho ? (double)st.far_culled32 / st.far_rho : 0.0);
     std::printf("FP32 pre-test skips %ld (exact distance not above sqrt(rn2): %ld)\n", st.fast32, st.fp32_bad);
+    std::printf("tiles %ld (all cells d_ok: %ld), (tile, source) pairs %ld, skipped %ld, unsound %ld, exact distance "
+                "not above sqrt(rn2): %ld; far pairs %ld skipped %ld (%.4f)\n", ts.tiles, ts.tiles_on, ts.pairs, ts.skipped,
+                ts.unsound, ts.dist_bad, ts.far_pairs, ts.far_skipped, ts.far_pairs ? (double)ts.far_skipped / ts.far_pairs : 0.0);
     const bool ok = st.unsound == 0 && st.det_bad == 0 && st.poly_bad == 0 && st.dok_bad == 0 && st.dok > 0 &&
+                    ts.unsound == 0 && ts.dist_bad == 0 && ts.skipped > 0 && ts.far_pairs > 0 &&
+                    ts.far_skipped == ts.far_pairs &&
                     st.hits > 0 && st.fp32_bad == 0 && st.fast32 > 0 &&
                     st.far_pairs > 0 && st.far_culled >= 0.999 * st.far_pairs &&
                     st.far_rho > 0 && st.far_culled32 >= 0.99 * st.far_rho;

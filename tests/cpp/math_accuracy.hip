@@ -17,6 +17,13 @@ __global__ void k(const double* y, const double* x, double* at, double* ap, doub
     ap[t] = atan2p_lt(y[t], fabs(x[t]));
     lg[t] = log_lt(fabs(x[t]));
 }
+// wave_max64 over wavefronts of random values (one in eight lanes NaN-free negative, the rest random
+// magnitudes), one result per lane
+__global__ void kmax(const double* v, double* out, int n)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    out[t] = wave_max64(v[t]);
+}
 static double ulp(double v) { return std::nextafter(std::fabs(v), INFINITY) - std::fabs(v); }
 int main()
 {
@@ -52,5 +59,17 @@ int main()
     printf("atan2_lt max error %.2f ulp, log_lt max error %.2f ulp (max abs %.2e), atan2p_lt max error %.2f ulp over %d "
            "inputs\n",
            wa, wl, wl_abs, wp, n);
+    // wave_max64: every lane of every wavefront gets that wavefront's maximum (n is a multiple of 256)
+    for (int i = 0; i < n; ++i) x[i] = (i % 8 == 3) ? -std::exp(le(g)) : std::copysign(std::exp(le(g)), u(g));
+    if (hipMemcpy(dx, x.data(), n * 8, hipMemcpyHostToDevice)) return 1;
+    hipLaunchKernelGGL(kmax, dim3(n / 256), dim3(256), 0, 0, dx, da, n);
+    if (hipMemcpy(at.data(), da, n * 8, hipMemcpyDeviceToHost)) return 1;
+    long bad = 0;
+    for (int w = 0; w < n / 64; ++w) {
+        double m = x[64 * w];
+        for (int l = 1; l < 64; ++l) m = std::fmax(m, x[64 * w + l]);
+        for (int l = 0; l < 64; ++l) bad += at[64 * w + l] != m;
+    }
+    printf("wave_max64 lanes not equal to their wavefront's maximum: %ld of %d\n", bad, n);
     return 0;
 }
