@@ -184,6 +184,15 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_delense(const double* __restrict
     const bool active = xid < n - 1 && yid < n - 1;
     const int64_t base = (int64_t)s * d.grid_stride;
     const int cell = yid * (n - 1) + xid;
+    // the first pass's sources, loaded before the cell set-up so that their latency overlaps it
+    const int cnt0 = min(SCAN_BLOCK, d.nsets);
+    double sx0 = 0., sy0 = 0., sdr0 = 0.;
+    if ((int)threadIdx.x < cnt0) {
+        const int64_t sk = (int64_t)s * d.nsets + threadIdx.x;
+        sx0 = d.src_x[sk];
+        sy0 = d.src_y[sk];
+        sdr0 = d.src_dr[sk];
+    }
     // lanes past the grid read the last cell of their row or column (in bounds) and record nothing
     const CellCorners c = load_corners(gx + base, gy + base, d, min(xid, n - 2), min(yid, n - 2));
     const double* sb = d.src_bound + 3 * (int64_t)s;
@@ -209,7 +218,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_delense(const double* __restrict
         __syncthreads();  // the previous pass is done with the staged sources
         if ((int)threadIdx.x < cnt) {
             const int64_t sk = (int64_t)s * d.nsets + k0 + threadIdx.x;
-            const double sx = d.src_x[sk], sy = d.src_y[sk], sdr = d.src_dr[sk];
+            const double sx = k0 ? d.src_x[sk] : sx0, sy = k0 ? d.src_y[sk] : sy0, sdr = k0 ? d.src_dr[sk] : sdr0;
             lsx[threadIdx.x] = sx;
             lsy[threadIdx.x] = sy;
             ldr[threadIdx.x] = sdr;
