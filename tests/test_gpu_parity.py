@@ -256,8 +256,8 @@ def test_map_fields_and_dpl(hip, oracle):
 
 @pytest.mark.parametrize("htype", [81, 8])
 def test_potential_next_to_the_centre(hip, oracle, htype):
-    """Pixels 1e-3 to 3e-2 core radii from a halo centre (never on it): the first logarithm of the
-    algebraic potential is formed as log1p of a small quantity, so the 1e-12 criterion holds there too
+    """Pixels 1e-3 to 3e-2 core radii from a halo centre (never on it): the potential's logarithms
+    are formed as log1p of small quantities, so the 1e-12 criterion holds there too
     (written as the logarithm of a quotient of nearly equal numbers it lost up to 1e-8 of the
     potential at such pixels)."""
     halos = [dict(type=htype, x=0.37, y=-0.21, sigma=420.0, e=0.35, theta=0.6, rcore=0.5, rcut=60.0),
@@ -303,7 +303,7 @@ def test_potential_lone_dpie_far_and_near(hip, oracle, epot):
 
 @pytest.mark.parametrize("which", ["mixed", "cluster"])
 def test_potential(hip, oracle, which):
-    """The potential (the algebraic form of the reference's pi05) against the extended oracle of the
+    """The potential (the deflection form of the reference's pi05) against the extended oracle of the
     reference's complex form, held to the FP64 criterion: 1e-12 of the sum over halos of |phi_h| at
     each pixel (the GPU's own error is below 1e-14 of it; no cancellation term is needed)."""
     if which == "mixed":
