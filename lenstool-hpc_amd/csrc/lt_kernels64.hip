@@ -55,6 +55,8 @@ __global__ __launch_bounds__(WAVE* ROWS) void k_hessian64(const HaloRec* __restr
             break;
         case TYPE_SIS:
             if (run.circ) {
+                // four halos per step: their records' scalar loads issue together, one wait per step
+#pragma unroll 4
                 for (int i = run.start; i < end; ++i) {
                     const HaloRec& h = halos[i];
 #pragma unroll
@@ -134,6 +136,8 @@ __global__ __launch_bounds__(WAVE* ROWS) void k_gradient64(const HaloRec* __rest
             break;
         default:  // TYPE_SIS
             if (run.circ) {
+                // four halos per step: their records' scalar loads issue together, one wait per step
+#pragma unroll 4
                 for (int i = run.start; i < end; ++i) {
                     const HaloRec& h = halos[i];
 #pragma unroll
