@@ -242,28 +242,24 @@ int lt_chi_batch_run(lt_chi_batch b, const lt_potential_soa* lenses, double* chi
     std::string err;
     // the previous run's upload may still read the staging area if that run stopped on an error
     LT_HIP_TRY(hipEventSynchronize(b->staged));
-    // a run uses the circular-SIS terms only when it is circular in every sample
-    for (Run& r : b->runs) r.circ = 1;
+    // halo types of every sample, and the circular-SIS flag of each run: set when every halo of the run
+    // is a circular SIS in every sample (the packer's per-sample rule, ANDed), known before any launch
+    for (Run& r : b->runs) r.circ = r.type == TYPE_SIS ? 1 : 0;
     for (int i = 0; i < b->S; ++i) {
         const lt_potential_soa& L = lenses[i];
         for (int h = 0; h < b->H; ++h)
             if (!L.type || L.type[h] != b->types[h])
                 return set_error(LT_ERR_INVALID, "sample " + std::to_string(i) + ": halo types differ from the batch's");
-        int st = pack_model(&L, b->H, Use::Gradient, recs, recs32, runs, err);
-        if (st) return set_error(st, "sample " + std::to_string(i) + ": " + err);
-        if (runs.size() != b->runs.size()) return set_error(LT_ERR_INVALID, "sample " + std::to_string(i) + ": run structure differs");
-        for (size_t r = 0; r < runs.size(); ++r) b->runs[r].circ &= runs[r].circ;
-        if (b->H) std::memcpy(&b->staging[(size_t)i * b->H], recs.data(), (size_t)b->H * sizeof(HaloRec));
+        for (Run& r : b->runs)
+            for (int h = r.start; h < r.start + r.count && r.circ; ++h)
+                r.circ = L.ellipticity_potential && L.ellipticity_potential[h] == 0.0;
     }
-    if (b->H)
-        LT_HIP_TRY(hipMemcpyAsync(b->recs, b->staging, (size_t)b->S * b->H * sizeof(HaloRec),
-                                  hipMemcpyHostToDevice, s));
-    LT_HIP_TRY(hipEventRecord(b->staged, s));
     if (!b->runs.empty())
         LT_HIP_TRY(hipMemcpyAsync(b->druns, b->runs.data(), b->runs.size() * sizeof(Run), hipMemcpyHostToDevice, s));
     int st = reset_counters(b, s);
     if (st) return st;
-    // gradient grids of every sample (src/chi_CPU_barycenter.cpp:125-137)
+    // gradient grids of every sample (src/chi_CPU_barycenter.cpp:125-137), launched per chunk of samples
+    // as soon as the chunk is packed, so that the host packs the next chunk while the device computes
     GridDesc g{};
     g.xmin = b->frame.xmin;
     g.ymin = b->frame.ymin;
@@ -274,17 +270,39 @@ int lt_chi_batch_run(lt_chi_batch b, const lt_potential_soa* lenses, double* chi
     g.pitch = b->n;
     g.halo_stride = b->H;
     g.plane_stride = (int64_t)b->n * b->n;
-    g.nsamples = b->S;
     OutDesc o{};
-    o.p0 = b->gx;
-    o.p1 = b->gy;
     o.kind = LT_MAP_GRADIENT;
     o.r = 1.0;
     o.mult = 1.0;
-    // 4 pixels per thread: with few halos per sample each halo record's scalar load is shared by more
-    // pixels (1024 samples of 11 halos on 512^2: 9.9 ms per batch step against 10.4 at 2; the planes
-    // are bit-identical for every pixels-per-thread)
-    LT_HIP_TRY(launch_gradient64(b->recs, b->druns, (int)b->runs.size(), g, o, 4, s));
+    const int chunk = std::max(64, (b->S + 7) / 8);
+    for (int c0 = 0; c0 < b->S; c0 += chunk) {
+        const int cn = std::min(chunk, b->S - c0);
+        for (int i = c0; i < c0 + cn; ++i) {
+            st = pack_model(&lenses[i], b->H, Use::Gradient, recs, recs32, runs, err);
+            if (!st && runs.size() != b->runs.size()) {
+                st = LT_ERR_INVALID;
+                err = "run structure differs";
+            }
+            if (st) {
+                // uploads of earlier chunks may still read the staging area: the next run waits for them
+                const hipError_t e = hipEventRecord(b->staged, s);
+                if (e != hipSuccess) return hip_error(e, "chi batch: event record");
+                return set_error(st, "sample " + std::to_string(i) + ": " + err);
+            }
+            if (b->H) std::memcpy(&b->staging[(size_t)i * b->H], recs.data(), (size_t)b->H * sizeof(HaloRec));
+        }
+        if (b->H)
+            LT_HIP_TRY(hipMemcpyAsync(b->recs + (size_t)c0 * b->H, b->staging + (size_t)c0 * b->H,
+                                      (size_t)cn * b->H * sizeof(HaloRec), hipMemcpyHostToDevice, s));
+        g.nsamples = cn;
+        o.p0 = b->gx + (size_t)c0 * g.plane_stride;
+        o.p1 = b->gy + (size_t)c0 * g.plane_stride;
+        // 4 pixels per thread: with few halos per sample each halo record's scalar load is shared by
+        // more pixels (1024 samples of 11 halos on 512^2: 9.9 ms per batch step against 10.4 at 2; the
+        // planes are bit-identical for every pixels-per-thread)
+        LT_HIP_TRY(launch_gradient64(b->recs + (size_t)c0 * b->H, b->druns, (int)b->runs.size(), g, o, 4, s));
+    }
+    LT_HIP_TRY(hipEventRecord(b->staged, s));
     LT_HIP_TRY(launch_points_gradient64(b->recs, b->druns, (int)b->runs.size(), b->d.img_x, b->d.img_y, b->nimg,
                                         b->S, b->H, b->img_gx, b->img_gy, s));
     LT_HIP_TRY(launch_barycentres(b->img_gx, b->img_gy, b->d, s));
