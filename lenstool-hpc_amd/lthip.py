@@ -695,10 +695,18 @@ class Model:
         check(lib().lt_model_update_batch(self.h, arr, len(lenses), C.c_void_p(stream)))
         self.lens = list(lenses)
 
-    def stage(self, lenses):
+    @staticmethod
+    def soa_array(lenses):
+        """The lt_potential_soa array of a list of lens models, for stage() (built once, it can be
+        staged many times without rebuilding it)."""
         lenses = lenses if isinstance(lenses, (list, tuple)) else [lenses]
-        arr = (M.LtPotentialSOA * len(lenses))(*[l.soa() for l in lenses])
-        check(lib().lt_model_stage(self.h, arr, len(lenses)))
+        return (M.LtPotentialSOA * len(lenses))(*[l.soa() for l in lenses])
+
+    def stage(self, lenses):
+        """Pack new sample parameters into the staging buffer: a list of lens models, or an array from
+        soa_array() (the lens models it was built from must stay alive)."""
+        arr = lenses if isinstance(lenses, C.Array) else self.soa_array(lenses)
+        check(lib().lt_model_stage(self.h, arr, len(arr)))
 
     def upload(self, stream=0):
         check(lib().lt_model_upload(self.h, C.c_void_p(stream)))
