@@ -202,6 +202,23 @@ __device__ __forceinline__ void hess_sis32_circ(const HaloRec32& h, float tx, fl
 // template tag of run32 for a run of circular SIS (not a reference type code)
 constexpr int TYPE_SIS_CIRC = -5;
 
+// For runs of SIS halos, the record fields their terms use, held in scalar registers from the top of
+// the halo's iteration (one empty asm statement that reads them all), so that their scalar loads issue
+// together and are waited for once; left to itself the compiler moves them into the branch after the
+// near-core test, once per pixel, each with its own wait, and an SIS term is too short to hide that
+// (Bayes maps of 1 dPIE + 10 circular SIS: 10.56 -> 10.41 ms). For PIEMD runs the same pinning
+// measured slower (8.95 -> 9.35 ms on the 503-dPIE map): their terms are long enough.
+template <int TYPE>
+__device__ __forceinline__ void pin_record(const HaloRec32& h, double xh, double yh)
+{
+    if constexpr (TYPE == TYPE_SIS_CIRC) {
+        asm volatile("" ::"s"(xh), "s"(yh), "s"(h.near2), "s"(h.hT));
+    } else if constexpr (TYPE == TYPE_SIS) {
+        asm volatile("" ::"s"(xh), "s"(yh), "s"(h.near2), "s"(h.hT), "s"(h.csx), "s"(h.snx), "s"(h.csy), "s"(h.sny),
+                     "s"(h.hc2), "s"(h.hs2m2), "s"(h.hs2h), "s"(h.sa), "s"(h.sb));
+    }
+}
+
 // One run of halos of one type: FP32 terms summed per chunk of CHUNK halos, FP64 terms (near
 // lanes) straight into the FP64 run sums. The records are wavefront-uniform scalar loads (the
 // Hessian keeps them in SGPRs: staged through LDS they cost 20 VGPRs, two waves of occupancy, and
@@ -217,8 +234,9 @@ __device__ __forceinline__ void run32(const HaloRec* __restrict__ h64, const Hal
         for (int k = 0; k < PX; ++k) fT[k] = fD[k] = fB[k] = 0.f;
         for (int i = c0; i < c1; ++i) {
             const HaloRec32& h = h32[i];
-            const double xh = h64[i].xh;
-            const Row32 rw = row32(h, h64[i].yh, py);
+            const double xh = h64[i].xh, yh = h64[i].yh;
+            pin_record<TYPE>(h, xh, yh);
+            const Row32 rw = row32(h, yh, py);
 #pragma unroll
             for (int k = 0; k < PX; ++k) {
                 const float tx = offset_x(xh, px[k]);
