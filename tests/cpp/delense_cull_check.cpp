@@ -14,7 +14,8 @@
 //      often as the FP64 margin;
 //   5. the tile pre-test (tile_reach / tile_filter32 over an 8 x 8 tile of cells, all with d_ok): whenever
 //      it skips a source, no cell of the tile has a hit and every cell's exact corner distance exceeds
-//      sqrt(rn2); on lens-like tiles it skips most sources that no cell of the tile is near.
+//      sqrt(rn2); on lens-like tiles it skips most sources that no cell of the tile is near;
+//   6. sqrt_up(q) is not below sqrt(q) (long double) over 1e-300 .. 1e300.
 // Cells come from smooth lens-like gradient fields, near-collinear and collapsed mapped triangles,
 // wild and huge gradients, sources on corners and edges, and NaN / infinity in gradients, sources and
 // D_ls/D_s. Exit status 0 = all checks hold; the counts are printed.
@@ -36,6 +37,8 @@ static q qdet(P2 a, P2 b, P2 c)
 {
     return ((q)b.x - a.x) * ((q)c.y - a.y) - ((q)b.y - a.y) * ((q)c.x - a.x);
 }
+
+static q q__sqrt(double v) { return (q)std::sqrt((long double)v); }
 
 static double amax(double a, double b) { return std::fabs(a) > std::fabs(b) ? std::fabs(a) : std::fabs(b); }
 
@@ -370,6 +373,12 @@ int main(int argc, char** argv)
     }
     TileStats ts;
     for (long t = 0; t < n / 64; ++t) check_tile(ts, rng);
+    // sqrt_up() bounds the square root from above, over magnitudes 1e-300 .. 1e300 (and 0)
+    long sqrt_bad = 0;
+    for (long t = 0; t < n / 8; ++t) {
+        const double qv = t == 0 ? 0.0 : std::pow(10.0, -300.0 + 600.0 * std::uniform_real_distribution<double>(0, 1)(rng));
+        if (!(q__sqrt(qv) <= (q)lt::sqrt_up(qv))) ++sqrt_bad;
+    }
     std::printf("trials %ld culled %ld hits %ld unsound %ld\n", st.trials, st.culled, st.hits, st.unsound);
     std::printf("tri_det error worst %.3g M^2 (bound 4.0e-15, over: %ld); d(dr) error worst %.3g M^2 (bound 3.2e-14, over: %ld)\n",
                 st.det_worst, st.det_bad, st.poly_worst, st.poly_bad);
@@ -382,7 +391,9 @@ int main(int argc, char** argv)
     std::printf("tiles %ld (all cells d_ok: %ld), (tile, source) pairs %ld, skipped %ld, unsound %ld, exact distance "
                 "not above sqrt(rn2): %ld; far pairs %ld skipped %ld (%.4f)\n", ts.tiles, ts.tiles_on, ts.pairs, ts.skipped,
                 ts.unsound, ts.dist_bad, ts.far_pairs, ts.far_skipped, ts.far_pairs ? (double)ts.far_skipped / ts.far_pairs : 0.0);
+    std::printf("sqrt_up below the square root: %ld of %ld\n", sqrt_bad, n / 8);
     const bool ok = st.unsound == 0 && st.det_bad == 0 && st.poly_bad == 0 && st.dok_bad == 0 && st.dok > 0 &&
+                    sqrt_bad == 0 &&
                     ts.unsound == 0 && ts.dist_bad == 0 && ts.skipped > 0 && ts.far_pairs > 0 &&
                     ts.far_skipped == ts.far_pairs &&
                     st.hits > 0 && st.fp32_bad == 0 && st.fast32 > 0 &&

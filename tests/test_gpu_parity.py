@@ -277,6 +277,28 @@ def test_potential_next_to_the_centre(hip, oracle, htype):
     print(f"[potential next to a type-{htype} centre] max error / tolerance {r:.3e}", flush=True)
 
 
+def test_potential_nan_exactly_on_a_centre(hip, oracle):
+    """A pixel exactly on a PIEMD or dPIE centre: the reference's potential is 0/0 there (NaN, as the
+    oracle's restatement gives); the kernel's form has the limit 0 there and returns NaN instead,
+    at that pixel only. The grid passes through both centres; an SIS centre stays finite (b0 R = 0)."""
+    halos = [dict(type=81, x=1.0, y=-2.0, sigma=500.0, e=0.3, theta=0.4, rcore=0.3, rcut=40.0),
+             dict(type=8, x=-3.0, y=2.0, sigma=300.0, e=0.2, theta=1.1, rcore=0.5, rcut=30.0),
+             dict(type=5, x=2.0, y=2.0, sigma=200.0, e=0.0, theta=0.0, rcore=0.0, rcut=0.0)]
+    lens = M.LensModel.from_halos(halos)
+    grid = M.Grid(M.grid_param(-4.0, 4.0, -4.0, 4.0), 9, 9)  # pixels on every integer point
+    print_case("potential on halo centres", lens, grid)
+    phi = hip.potential_grid(lens, grid)
+    ext = oracle.potential_grid(lens, grid, extended=True)
+    nan = ~np.isfinite(phi)
+    assert nan.sum() == 2 and nan[2, 5] and nan[6, 1]  # rows y = -2 and y = 2, columns x = 1 and x = -3
+    np.testing.assert_array_equal(~np.isfinite(ext), nan)
+    S = np.zeros_like(ext)
+    for i in range(lens.n):
+        S += np.abs(oracle.potential_grid(lens.subset([i]), grid, extended=True))
+    fin = ~nan
+    assert_close("phi", phi[fin], ext[fin], S[fin], np.zeros(fin.sum()))
+
+
 @pytest.mark.parametrize("epot", [0.3, 0.6])
 def test_potential_lone_dpie_far_and_near(hip, oracle, epot):
     """One dPIE alone (so the criterion is 1e-12 of its own |phi|): pixels 30 to 300 cut radii out,
