@@ -111,9 +111,9 @@ __device__ __forceinline__ Frame32 frame32(const HaloRec32& h, const Row32& r, f
     return f;
 }
 
-// numerators of xx, xy, yy and their common factor 1 / |Q|^2
+// numerators of xx - yy and xy, their common factor 1 / |Q|^2, and the trace xx + yy = 1 / W (lt_terms64.h)
 struct Q32 {
-    float xx, xy, yy, iq;
+    float dd, xy, iq, iW;
 };
 
 // pixel-only quantities shared by the core and cut terms (the scaled form of lt_terms64.h)
@@ -142,10 +142,10 @@ __device__ __forceinline__ Q32 q_term(const HaloRec32& h, const QShared& s, floa
     const float Qr = W + r;
     const float U = __builtin_fmaf(s.Y, iW, h.k5);
     Q32 o;
-    o.xx = __builtin_fmaf(-s.A, iW, Qr);
+    o.dd = -__builtin_fmaf(s.A, iW, U * __builtin_fmaf(h.k3, W, -s.ky4));
     o.xy = __builtin_fmaf(-s.kx2, U, s.Qi);
-    o.yy = __builtin_fmaf(U, __builtin_fmaf(h.k3, W, -s.ky4), Qr);
     o.iq = rcp32(__builtin_fmaf(Qr, Qr, s.Qi2));
+    o.iW = iW;
     return o;
 }
 
@@ -164,18 +164,13 @@ __device__ __forceinline__ void hess_piemd32(const HaloRec32& h, const Frame32& 
 {
     const QShared s = q_shared(h, f.x, f.y);
     const Q32 a = q_term(h, s, h.ra, h.ra2c);
-    float xx, xy, yy;
     if (CUT) {
         const Q32 b = q_term(h, s, h.rb, h.rb2c);
-        xx = __builtin_fmaf(a.xx, a.iq, -(b.xx * b.iq));
-        xy = __builtin_fmaf(a.xy, a.iq, -(b.xy * b.iq));
-        yy = __builtin_fmaf(a.yy, a.iq, -(b.yy * b.iq));
+        acc_lab(h, a.iW - b.iW, __builtin_fmaf(a.dd, a.iq, -(b.dd * b.iq)), __builtin_fmaf(a.xy, a.iq, -(b.xy * b.iq)),
+                fT, fD, fB);
     } else {
-        xx = a.xx * a.iq;
-        xy = a.xy * a.iq;
-        yy = a.yy * a.iq;
+        acc_lab(h, a.iW, a.dd * a.iq, a.xy * a.iq, fT, fD, fB);
     }
-    acc_lab(h, xx + yy, xx - yy, xy, fT, fD, fB);
 }
 
 // SIS / SIE: halo-frame Hessian amph / R^3 (y^2, -xy, x^2), R^2 = (1-e) x^2 + (1+e) y^2
@@ -470,18 +465,18 @@ __device__ __forceinline__ void add_term32(const HaloRec32& h, const Seg32& s, f
     } else {
         const QShared q = q_shared(h, X, Y);
         const Q32 a = q_term(h, q, h.ra, h.ra2c);
-        float xx, xy, yy;
+        float T, D, B;
         if constexpr (TYPE == TYPE_DPIE) {
             const Q32 b = q_term(h, q, h.rb, h.rb2c);
-            xx = __builtin_fmaf(a.xx, a.iq, -(b.xx * b.iq));
-            xy = __builtin_fmaf(a.xy, a.iq, -(b.xy * b.iq));
-            yy = __builtin_fmaf(a.yy, a.iq, -(b.yy * b.iq));
+            T = a.iW - b.iW;
+            D = __builtin_fmaf(a.dd, a.iq, -(b.dd * b.iq));
+            B = __builtin_fmaf(a.xy, a.iq, -(b.xy * b.iq));
         } else {
-            xx = a.xx * a.iq;
-            xy = a.xy * a.iq;
-            yy = a.yy * a.iq;
+            T = a.iW;
+            D = a.dd * a.iq;
+            B = a.xy * a.iq;
         }
-        acc_lab(h, far ? xx + yy : 0.f, far ? xx - yy : 0.f, far ? xy : 0.f, aT, aD, aB);
+        acc_lab(h, far ? T : 0.f, far ? D : 0.f, far ? B : 0.f, aT, aD, aB);
     }
 }
 

@@ -47,10 +47,12 @@ __device__ __forceinline__ PixelFrame halo_frame(const HaloRec& h, double px, do
 //   phi_xx ~ (Re Q - cx1 a^2 x^2 / w) / |Q|^2,  phi_xy ~ x (2 s a^2 - cx1 U) / |Q|^2,
 //   phi_yy ~ (Re Q + U t) / |Q|^2.
 // Re Q >= r > 0: unlike the closed form, whose two pieces are both singular at halo-frame
-// (0, 2 r s) and cancel there, nothing cancels anywhere. Returned: the numerators of xx, xy, yy
-// and |Q|^2 (the caller forms the reciprocals: one for both radii of a dPIE).
+// (0, 2 r s) and cancel there, nothing cancels anywhere. The trace needs no quotient: (xx + yy) / |Q|^2
+// = 1 / W exactly (the term's convergence, elliptical in w; with the scaled W below), so only xx - yy
+// and xy are divided by |Q|^2. Returned: the numerators of xx - yy and xy, |Q|^2 and 1 / W (the
+// caller forms the reciprocals of |Q|^2: one for both radii of a dPIE).
 struct PiemdQ {
-    double xx, xy, yy, q2;
+    double dd, xy, q2, iW;
 };
 
 // Evaluated in scaled halo-frame coordinates X = cx1 a x, Y = cx1 b y (the scales are folded into
@@ -58,7 +60,7 @@ struct PiemdQ {
 // Re Q = W + r, and with U' = U / b = Y / W + 2 s a,
 //   xx = Re Q - X^2 / W,  xy = k1 X - k2 X U',  yy = Re Q + U' (k3 W - k4 Y),  Im Q = k1 X,
 // k1 = 2 s a / cx1, k2 = b / a, k3 = 2 s b / cx1, k4 = 1 / cx1^2, k5 = 2 s a: every pixel-dependent
-// quantity is one operation from X, Y and W.
+// quantity is one operation from X, Y and W. xx - yy = -(X^2 / W + U' (k3 W - k4 Y)): Re Q drops out.
 struct PiemdShared {
     double wq0, Qi2, A, Y, Qi, kx2, ky4;
 };
@@ -87,10 +89,10 @@ __device__ __forceinline__ PiemdQ piemd_q_term(const HaloRec& h, const PiemdShar
     const double Qr = W + r;
     const double U = __builtin_fma(s.Y, iW, h.k5);
     PiemdQ o;
-    o.xx = __builtin_fma(-s.A, iW, Qr);
+    o.dd = -__builtin_fma(s.A, iW, U * __builtin_fma(h.k3, W, -s.ky4));
     o.xy = __builtin_fma(-s.kx2, U, s.Qi);
-    o.yy = __builtin_fma(U, __builtin_fma(h.k3, W, -s.ky4), Qr);
     o.q2 = __builtin_fma(Qr, Qr, s.Qi2);
+    o.iW = iW;
     return o;
 }
 
@@ -100,23 +102,22 @@ __device__ __forceinline__ void hess_piemd(const HaloRec& h, double px, double p
 {
     const PiemdShared s = piemd_shared(h, px, py);
     const PiemdQ a = piemd_q_term(h, s, h.rc, h.ra2c);
-    double xx, xy, yy;
+    double dT, dD, dB;
     if (CUT) {
         const PiemdQ b = piemd_q_term(h, s, h.rcut, h.rb2c);
         // one reciprocal for both radii: 1/|Qa|^2 = |Qb|^2 / (|Qa|^2 |Qb|^2), and the same for b
         const double ip = rcp64_n2(a.q2 * b.q2);
         const double ia = b.q2 * ip, ib = a.q2 * ip;
-        xx = __builtin_fma(a.xx, ia, -(b.xx * ib));
-        xy = __builtin_fma(a.xy, ia, -(b.xy * ib));
-        yy = __builtin_fma(a.yy, ia, -(b.yy * ib));
+        dT = a.iW - b.iW;
+        dD = __builtin_fma(a.dd, ia, -(b.dd * ib));
+        dB = __builtin_fma(a.xy, ia, -(b.xy * ib));
     } else {
         const double ia = rcp64_n2(a.q2);
-        xx = a.xx * ia;
-        xy = a.xy * ia;
-        yy = a.yy * ia;
+        dT = a.iW;
+        dD = a.dd * ia;
+        dB = a.xy * ia;
     }
-    // xx, xy, yy are the numerators of the unscaled halo-frame derivatives (the scaling cancels)
-    const double dT = xx + yy, dD = xx - yy, dB = xy;
+    // dT, dD, dB: xx + yy, xx - yy and xy of the unscaled halo-frame derivatives (the scaling cancels)
     aT = __builtin_fma(h.qT, dT, aT);
     aD = __builtin_fma(h.qc2, dD, __builtin_fma(h.qs2m2, dB, aD));
     aB = __builtin_fma(h.qs2h, dD, __builtin_fma(h.qc2, dB, aB));
